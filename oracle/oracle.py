@@ -1,0 +1,329 @@
+"""ctypes view of the oracle (oracle.c) and of the reference-operator harness
+(ref_harness.cpp, built into oracle/_ref when the exafmm-beta tree is present).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.  Arrays are numpy; complex
+expansions are numpy complex128 of shape (ncell, NTERM).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libexafmm_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def build(quiet=True):
+    """Compile the oracle (and _ref when the reference tree is present)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    return ctypes.CDLL(path)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = _load(ORACLE_SO)
+        L.orc_kernel_create.restype = _vp
+        L.orc_kernel_create.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_kernel_destroy.argtypes = [_vp]
+        L.orc_kernel_nterm.argtypes = [_vp]
+        L.orc_p2p.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_p2m.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_m2m.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        L.orc_m2l.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_l2l.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        L.orc_l2p.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_bounds.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_tree_build.restype = _vp
+        L.orc_tree_build.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_tree_destroy.argtypes = [_vp]
+        L.orc_tree_ncell.argtypes = [_vp]
+        L.orc_tree_depth.argtypes = [_vp]
+        L.orc_tree_get.argtypes = [_vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_traverse.restype = _vp
+        L.orc_traverse.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_lists_destroy.argtypes = [_vp]
+        L.orc_lists_count.restype = ctypes.c_int64
+        L.orc_lists_count.argtypes = [_vp, ctypes.c_int]
+        L.orc_lists_get.argtypes = [_vp, _ip, _ip, _ip]
+        L.orc_fmm.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_eval_lists.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_int64, _ip, _ip, _ip, _dp,
+                                     _dp, _dp]
+        L.orc_upward.argtypes = [_vp, _vp, _dp, _dp, _dp]
+        L.orc_downward.argtypes = [_vp, _vp, _dp, _dp, _dp]
+        L.orc_direct.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_rel_l2.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_data_cube.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_data_plummer.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_data_source.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_sample_indices.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = _load(REF_SO)
+        R.ref_kernel_create.restype = _vp
+        R.ref_kernel_create.argtypes = [ctypes.c_int, ctypes.c_double]
+        R.ref_kernel_destroy.argtypes = [_vp]
+        R.ref_p2p.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        R.ref_p2m.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        R.ref_m2m.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        R.ref_m2l.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+        R.ref_l2l.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        R.ref_l2p.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp, _dp]
+        R.ref_eval.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
+                               _dp, _dp, ctypes.c_long, _ip, _ip, _ip, _dp, _dp, _dp]
+        R.ref_init_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        _ref = R
+    return _ref
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Kernel:
+    """Oracle Kernel(P): the six operators of kernels/laplace.h (or, with
+    use_ref=True, the reference's own compiled operators)."""
+
+    def __init__(self, P, eps2=0.0, use_ref=False):
+        self.P = P
+        self.nterm = P * (P + 1) // 2
+        self.use_ref = use_ref
+        if use_ref:
+            self._L = ref()
+            self.h = self._L.ref_kernel_create(P, eps2)
+            self._pre = "ref_"
+        else:
+            self._L = lib()
+            self.h = self._L.orc_kernel_create(P, eps2)
+            self._pre = "orc_"
+
+    def __del__(self):
+        try:
+            getattr(self._L, self._pre + "kernel_destroy")(self.h)
+        except Exception:
+            pass
+
+    def _f(self, name):
+        return getattr(self._L, self._pre + name)
+
+    def p2p(self, xi, xj, qj, xper=None, trg=None):
+        xi, xj, qj = _c(xi), _c(xj), _c(qj)
+        trg = np.zeros((len(xi), 4)) if trg is None else trg
+        xp = _c(xper if xper is not None else np.zeros(3))
+        self._f("p2p")(self.h, len(xi), _ptr(xi), _ptr(trg), len(xj), _ptr(xj), _ptr(qj), _ptr(xp))
+        return trg
+
+    def p2m(self, Xc, x, q, M=None):
+        M = np.zeros(self.nterm, np.complex128) if M is None else M
+        x, q = _c(x), _c(q)
+        self._f("p2m")(self.h, _ptr(_c(Xc)), len(x), _ptr(x), _ptr(q), _ptr(M.view(np.float64)))
+        return M
+
+    def m2m(self, Xp, Xc, Mc, Mp=None):
+        Mp = np.zeros(self.nterm, np.complex128) if Mp is None else Mp
+        Mc = _c(Mc, np.complex128)
+        self._f("m2m")(self.h, _ptr(_c(Xp)), _ptr(_c(Xc)), _ptr(Mc.view(np.float64)),
+                       _ptr(Mp.view(np.float64)))
+        return Mp
+
+    def m2l(self, Xi, Xj, Mj, Li=None, xper=None):
+        Li = np.zeros(self.nterm, np.complex128) if Li is None else Li
+        Mj = _c(Mj, np.complex128)
+        xp = _c(xper if xper is not None else np.zeros(3))
+        self._f("m2l")(self.h, _ptr(_c(Xi)), _ptr(_c(Xj)), _ptr(xp), _ptr(Mj.view(np.float64)),
+                       _ptr(Li.view(np.float64)))
+        return Li
+
+    def l2l(self, Xc, Xp, Lp, Lc=None):
+        Lc = np.zeros(self.nterm, np.complex128) if Lc is None else Lc
+        Lp = _c(Lp, np.complex128)
+        self._f("l2l")(self.h, _ptr(_c(Xc)), _ptr(_c(Xp)), _ptr(Lp.view(np.float64)),
+                       _ptr(Lc.view(np.float64)))
+        return Lc
+
+    def l2p(self, Xc, L, x, trg=None):
+        x = _c(x)
+        trg = np.zeros((len(x), 4)) if trg is None else trg
+        L = _c(L, np.complex128)
+        self._f("l2p")(self.h, _ptr(_c(Xc)), _ptr(L.view(np.float64)), len(x), _ptr(x), _ptr(trg))
+        return trg
+
+
+def bounds(x):
+    x = _c(x)
+    lo, hi = np.zeros(3), np.zeros(3)
+    lib().orc_bounds(len(x), _ptr(x), _ptr(lo), _ptr(hi))
+    return lo, hi
+
+
+class Tree:
+    """BuildTree::buildTree output: perm (tree order -> input index) and the
+    CellBase fields."""
+
+    def __init__(self, x, ncrit=64, bnds=None):
+        L = lib()
+        x = _c(x)
+        self.n = len(x)
+        lo, hi = bnds if bnds is not None else bounds(x)
+        self.h = L.orc_tree_build(self.n, _ptr(x), ncrit, _ptr(_c(lo)), _ptr(_c(hi)))
+        nc = L.orc_tree_ncell(self.h)
+        self.ncell = nc
+        self.depth = L.orc_tree_depth(self.h)
+        self.perm = np.zeros(self.n, np.int32)
+        self.iparent, self.ichild, self.nchild, self.ibody, self.nbody = (
+            np.zeros(nc, np.int32) for _ in range(5))
+        self.X = np.zeros((nc, 3))
+        self.R = np.zeros(nc)
+        self.icell = np.zeros(nc, np.uint64)
+        L.orc_tree_get(self.h, _ptr(self.perm, _ip), _ptr(self.iparent, _ip),
+                       _ptr(self.ichild, _ip), _ptr(self.nchild, _ip), _ptr(self.ibody, _ip),
+                       _ptr(self.nbody, _ip), _ptr(self.X), _ptr(self.R),
+                       self.icell.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+
+    def __del__(self):
+        try:
+            lib().orc_tree_destroy(self.h)
+        except Exception:
+            pass
+
+    def traverse(self, theta=0.4, nspawn=5000, xper=None):
+        L = lib()
+        xp = _c(xper if xper is not None else np.zeros(3))
+        h = L.orc_traverse(self.h, theta, nspawn, _ptr(xp))
+        n = L.orc_lists_count(h, -1)
+        ci, cj, ty = (np.zeros(n, np.int32) for _ in range(3))
+        L.orc_lists_get(h, _ptr(ci, _ip), _ptr(cj, _ip), _ptr(ty, _ip))
+        L.orc_lists_destroy(h)
+        return ci, cj, ty
+
+    def fmm(self, kernel, x_sorted, q_sorted, theta=0.4, nspawn=5000, want_ml=False):
+        L = lib()
+        xs, qs = _c(x_sorted), _c(q_sorted)
+        trg = np.zeros((self.n, 4))
+        M = Lc = None
+        if want_ml:
+            M = np.zeros((self.ncell, kernel.nterm), np.complex128)
+            Lc = np.zeros((self.ncell, kernel.nterm), np.complex128)
+        L.orc_fmm(kernel.h, self.h, _ptr(xs), _ptr(qs), theta, nspawn, _ptr(trg),
+                  _ptr(M.view(np.float64)) if want_ml else None,
+                  _ptr(Lc.view(np.float64)) if want_ml else None)
+        return (trg, M, Lc) if want_ml else trg
+
+    def upward(self, kernel, x_sorted, q_sorted):
+        M = np.zeros((self.ncell, kernel.nterm), np.complex128)
+        xs, qs = _c(x_sorted), _c(q_sorted)
+        lib().orc_upward(kernel.h, self.h, _ptr(xs), _ptr(qs), _ptr(M.view(np.float64)))
+        return M
+
+    def eval_lists(self, kernel, x_sorted, q_sorted, ci, cj, ty, M, trg=None, Lc=None):
+        xs, qs = _c(x_sorted), _c(q_sorted)
+        trg = np.zeros((self.n, 4)) if trg is None else trg
+        Lc = np.zeros((self.ncell, kernel.nterm), np.complex128) if Lc is None else Lc
+        ci, cj, ty = (_c(a, np.int32) for a in (ci, cj, ty))
+        M = _c(M, np.complex128)
+        lib().orc_eval_lists(kernel.h, self.h, _ptr(xs), _ptr(qs), len(ci), _ptr(ci, _ip),
+                             _ptr(cj, _ip), _ptr(ty, _ip), _ptr(trg), _ptr(Lc.view(np.float64)),
+                             _ptr(M.view(np.float64)))
+        return trg, Lc
+
+    def downward(self, kernel, x_sorted, Lc, trg):
+        xs = _c(x_sorted)
+        lib().orc_downward(kernel.h, self.h, _ptr(xs), _ptr(Lc.view(np.float64)), _ptr(trg))
+        return trg
+
+    def ref_eval(self, ref_kernel, x_sorted, q_sorted, ci, cj, ty, want_ml=False):
+        """The reference's own operators over this tree and pair list."""
+        R = ref()
+        xs, qs = _c(x_sorted), _c(q_sorted)
+        trg = np.zeros((self.n, 4))
+        nt = ref_kernel.nterm
+        M = np.zeros((self.ncell, nt), np.complex128)
+        Lc = np.zeros((self.ncell, nt), np.complex128)
+        ci, cj, ty = (_c(a, np.int32) for a in (ci, cj, ty))
+        R.ref_eval(ref_kernel.h, self.n, _ptr(xs), _ptr(qs), self.ncell, _ptr(self.iparent, _ip),
+                   _ptr(self.ichild, _ip), _ptr(self.nchild, _ip), _ptr(self.ibody, _ip),
+                   _ptr(self.nbody, _ip), _ptr(_c(self.X)), _ptr(self.R), len(ci), _ptr(ci, _ip),
+                   _ptr(cj, _ip), _ptr(ty, _ip), _ptr(trg), _ptr(M.view(np.float64)),
+                   _ptr(Lc.view(np.float64)))
+        return (trg, M, Lc) if want_ml else trg
+
+
+def direct(kernel, xi, xj, qj):
+    xi, xj, qj = _c(xi), _c(xj), _c(qj)
+    trg = np.zeros((len(xi), 4))
+    lib().orc_direct(kernel.h, len(xi), _ptr(xi), _ptr(trg), len(xj), _ptr(xj), _ptr(qj))
+    return trg
+
+
+def rel_l2(trg_ref, trg):
+    a, b = _c(trg_ref), _c(trg)
+    p, q = ctypes.c_double(), ctypes.c_double()
+    lib().orc_rel_l2(len(a), _ptr(a), _ptr(b), ctypes.byref(p), ctypes.byref(q))
+    return p.value, q.value
+
+
+def data_cube(n, seed=0):
+    x = np.zeros((n, 3))
+    lib().orc_data_cube(n, seed, _ptr(x))
+    return x
+
+
+def data_plummer(n, seed=0):
+    x = np.zeros((n, 3))
+    lib().orc_data_plummer(n, seed, _ptr(x))
+    return x
+
+
+def data_source(n, seed=0):
+    q = np.zeros(n)
+    lib().orc_data_source(n, seed, _ptr(q))
+    return q
+
+
+def sample_indices(n, ntarget):
+    idx = np.zeros(max(n, 1), np.int32)
+    k = lib().orc_sample_indices(n, ntarget, _ptr(idx, _ip))
+    return idx[:k]
+
+
+def ref_init_bodies(n, dist="cube", seed=0):
+    x = np.zeros((n, 3))
+    q = np.zeros(n)
+    ref().ref_init_bodies(n, dist.encode(), seed, _ptr(x), _ptr(q))
+    return x, q

@@ -1,0 +1,211 @@
+// ref_harness.cpp -- drives exafmm-beta's own Laplace operators (kernels/laplace.h)
+// and input generators (include/dataset.h), compiled from the reference tree
+// in place by oracle/Makefile into oracle/_ref/libexafmm_ref.so.
+//
+// TEST INFRASTRUCTURE ONLY.  This file contains no reference code: it includes
+// the reference headers from the reference tree and calls them.  The
+// reference's tree, traversal and up/down pass headers pull in the generated
+// config.h (include/thread.h:3) and cannot be built here without the
+// reference's build system, so the pass order of include/up_down_pass.h:14-75
+// is re-driven below over a tree and pair lists supplied by the caller.
+#include "types.h"
+#include "kernel.h"
+#include "dataset.h"
+#include <cstring>
+
+using namespace exafmm_laplace;
+
+namespace {
+struct RefCtx {
+  Kernel kernel;
+  explicit RefCtx(int P, double eps2) : kernel(P, eps2, complex_t(1., .1)) {}
+};
+
+void load_bodies(Bodies &b, int n, const double *x, const double *q) {
+  b.resize(n);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) b[i].X[d] = x[3 * i + d];
+    b[i].SRC = q ? q[i] : 0.0;
+    b[i].TRG = 0;
+    b[i].IBODY = i;
+    b[i].ICELL = 0;
+    b[i].WEIGHT = 1;
+  }
+}
+
+void set_cell(Cell &c, const double *X, double R, int nterm) {
+  for (int d = 0; d < 3; d++) c.X[d] = X[d];
+  c.R = R;
+  c.IPARENT = 0;
+  c.ICHILD = 0;
+  c.NCHILD = 0;
+  c.IBODY = 0;
+  c.NBODY = 0;
+  c.M.assign(nterm, 0.0);
+  c.L.assign(nterm, 0.0);
+}
+
+void copy_in(std::vector<complex_t> &v, const double *a, int nterm) {
+  for (int i = 0; i < nterm; i++) v[i] = complex_t(a[2 * i], a[2 * i + 1]);
+}
+void add_out(const std::vector<complex_t> &v, double *a, int nterm) {
+  for (int i = 0; i < nterm; i++) {
+    a[2 * i] += v[i].real();
+    a[2 * i + 1] += v[i].imag();
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void *ref_kernel_create(int P, double eps2) { return new RefCtx(P, eps2); }
+void ref_kernel_destroy(void *k) { delete static_cast<RefCtx *>(k); }
+
+// Kernel::P2P (SIMD + scalar tail), trg accumulated
+void ref_p2p(void *kp, int ni, const double *xi, double *trgi, int nj, const double *xj,
+             const double *qj, const double *xper) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies bi, bj;
+  load_bodies(bi, ni, xi, nullptr);
+  load_bodies(bj, nj, xj, qj);
+  Cells cells(2);
+  cells[0].BODY = bi.begin();
+  cells[0].NBODY = ni;
+  cells[1].BODY = bj.begin();
+  cells[1].NBODY = nj;
+  for (int d = 0; d < 3; d++) K.Xperiodic[d] = xper ? xper[d] : 0.0;
+  K.P2P(cells.begin(), cells.begin() + 1);
+  K.Xperiodic = 0;
+  for (int i = 0; i < ni; i++)
+    for (int c = 0; c < 4; c++) trgi[4 * i + c] += bi[i].TRG[c];
+}
+
+void ref_p2m(void *kp, const double *Xc, int n, const double *x, const double *q, double *M) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies b;
+  load_bodies(b, n, x, q);
+  Cells cells(1);
+  set_cell(cells[0], Xc, 1.0, K.NTERM);
+  cells[0].BODY = b.begin();
+  cells[0].NBODY = n;
+  K.P2M(cells.begin());
+  add_out(cells[0].M, M, K.NTERM);
+}
+
+void ref_m2m(void *kp, const double *Xp, const double *Xc, const double *Mc, double *Mp) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Cells cells(2);
+  set_cell(cells[0], Xp, 2.0, K.NTERM);
+  set_cell(cells[1], Xc, 1.0, K.NTERM);
+  copy_in(cells[1].M, Mc, K.NTERM);
+  cells[0].ICHILD = 1;
+  cells[0].NCHILD = 1;
+  K.M2M(cells.begin(), cells.begin());
+  add_out(cells[0].M, Mp, K.NTERM);
+}
+
+void ref_m2l(void *kp, const double *Xi, const double *Xj, const double *xper, const double *Mj,
+             double *Li) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Cells cells(2);
+  set_cell(cells[0], Xi, 1.0, K.NTERM);
+  set_cell(cells[1], Xj, 1.0, K.NTERM);
+  copy_in(cells[1].M, Mj, K.NTERM);
+  for (int d = 0; d < 3; d++) K.Xperiodic[d] = xper ? xper[d] : 0.0;
+  K.M2L(cells.begin(), cells.begin() + 1);
+  K.Xperiodic = 0;
+  add_out(cells[0].L, Li, K.NTERM);
+}
+
+void ref_l2l(void *kp, const double *Xc, const double *Xp, const double *Lp, double *Lc) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Cells cells(2);
+  set_cell(cells[0], Xp, 2.0, K.NTERM);
+  set_cell(cells[1], Xc, 1.0, K.NTERM);
+  copy_in(cells[0].L, Lp, K.NTERM);
+  cells[1].IPARENT = 0;
+  K.L2L(cells.begin() + 1, cells.begin());
+  add_out(cells[1].L, Lc, K.NTERM);
+}
+
+void ref_l2p(void *kp, const double *Xc, const double *L, int n, const double *x, double *trg) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies b;
+  load_bodies(b, n, x, nullptr);
+  Cells cells(1);
+  set_cell(cells[0], Xc, 1.0, K.NTERM);
+  copy_in(cells[0].L, L, K.NTERM);
+  cells[0].BODY = b.begin();
+  cells[0].NBODY = n;
+  K.L2P(cells.begin());
+  for (int i = 0; i < n; i++)
+    for (int c = 0; c < 4; c++) trg[4 * i + c] += b[i].TRG[c];
+}
+
+// Full evaluation with the reference operators over a caller-supplied tree
+// (CellBase fields, bodies in tree order) and pair list in visiting order.
+// Pass order follows include/up_down_pass.h:14-75.
+struct Pass {
+  Kernel &K;
+  Cells &cells;
+  void post(C_iter C) {
+    C_iter C0 = cells.begin();
+    for (C_iter CC = C0 + C->ICHILD; CC != C0 + C->ICHILD + C->NCHILD; CC++) post(CC);
+    if (C->NCHILD == 0) K.P2M(C);
+    else K.M2M(C, C0);
+  }
+  void pre(C_iter C) {
+    C_iter C0 = cells.begin();
+    K.L2L(C, C0);
+    if (C->NCHILD == 0) K.L2P(C);
+    for (C_iter CC = C0 + C->ICHILD; CC != C0 + C->ICHILD + C->NCHILD; CC++) pre(CC);
+  }
+};
+
+void ref_eval(void *kp, int n, const double *x, const double *q, int ncell, const int *iparent,
+              const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+              const double *X, const double *R, long npair, const int *ci, const int *cj,
+              const int *type, double *trg, double *Mout, double *Lout) {
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies bodies;
+  load_bodies(bodies, n, x, q);
+  Cells cells(ncell);
+  for (int c = 0; c < ncell; c++) {
+    set_cell(cells[c], X + 3 * c, R[c], K.NTERM);
+    cells[c].IPARENT = iparent[c];
+    cells[c].ICHILD = ichild[c];
+    cells[c].NCHILD = nchild[c];
+    cells[c].IBODY = ibody[c];
+    cells[c].NBODY = nbody[c];
+    cells[c].BODY = bodies.begin() + ibody[c];
+  }
+  if (ncell == 0) return;
+  Pass pass{K, cells};
+  pass.post(cells.begin());
+  K.Xperiodic = 0;
+  for (long p = 0; p < npair; p++) {
+    if (type[p] == 0) K.M2L(cells.begin() + ci[p], cells.begin() + cj[p]);
+    else K.P2P(cells.begin() + ci[p], cells.begin() + cj[p]);
+  }
+  C_iter C0 = cells.begin();
+  if (C0->NCHILD == 0) K.L2P(C0);
+  for (C_iter CC = C0 + C0->ICHILD; CC != C0 + C0->ICHILD + C0->NCHILD; CC++) pass.pre(CC);
+  for (int i = 0; i < n; i++)
+    for (int c = 0; c < 4; c++) trg[4 * i + c] += bodies[i].TRG[c];
+  if (Mout)
+    for (int c = 0; c < ncell; c++) add_out(cells[c].M, Mout + 2 * (size_t)c * K.NTERM, K.NTERM);
+  if (Lout)
+    for (int c = 0; c < ncell; c++) add_out(cells[c].L, Lout + 2 * (size_t)c * K.NTERM, K.NTERM);
+}
+
+// Dataset::initBodies (include/dataset.h:230-255): positions + initSource.
+void ref_init_bodies(int n, const char *dist, int seed, double *x, double *q) {
+  Dataset data;
+  Bodies b = data.initBodies(n, dist, seed);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) x[3 * i + d] = b[i].X[d];
+    q[i] = b[i].SRC;
+  }
+}
+
+}  // extern "C"
