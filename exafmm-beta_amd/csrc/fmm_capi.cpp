@@ -1,0 +1,593 @@
+// fmm_capi.cpp -- implementation of the C ABI in include/exafmm_amd.h.
+#include "exafmm_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fmm_device.hpp"
+#include "fmm_host.hpp"
+
+namespace {
+
+// Grow-only device buffer: the arena stays resident across calls so repeated
+// solves (successive MD steps, served requests) do not reallocate.
+struct DBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+std::mutex g_init_mu;
+bool g_init_done[64] = {};
+
+}  // namespace
+
+struct fmm_ctx {
+  fmm_params p{};
+  std::string err;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[5] = {};
+  fmm::Tree tree;
+  fmm::Lists lists;
+  int n = 0;
+  bool have_bodies = false;
+  fmm_stats stats{};
+  // host-side level tables
+  std::vector<int> internal_ptr;  // per level offsets into internal cell list
+  std::vector<int> level_ptr;     // per level offsets into level_cells
+  // device buffers
+  DBuf x, y, z, q, pot, ax, ay, az;
+  DBuf cx, cy, cz, ibody, nbody, ichild, nchild, iparent;
+  DBuf M, L;
+  DBuf leaves, internal, level_cells, body_leaf;
+  DBuf m2l_ptr, m2l_src, m2l_tgt, p2p_ptr, p2p_src, p2p_tgt;
+};
+
+struct fmm_plan {
+  int n = 0;
+  fmm::Tree tree;
+  fmm::Lists lists;
+  double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
+  double t_bounds = 0, t_tree = 0, t_lists = 0;
+};
+
+#define FMM_FAIL(code, msg) \
+  do {                      \
+    c->err = (msg);         \
+    return (code);          \
+  } while (0)
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      c->err = std::string(#expr) + " failed: " + hipGetErrorString(e_);           \
+      return e_ == hipErrorOutOfMemory ? FMM_ERR_MEMORY : FMM_ERR_DEVICE;          \
+    }                                                                              \
+  } while (0)
+
+namespace {
+
+int ensure(fmm_ctx *c, DBuf &b, size_t bytes) {
+  if (bytes <= b.cap && b.p) return FMM_OK;
+  if (b.p) {
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t sz = bytes ? bytes : 16;
+  HIPCHK(hipMalloc(&b.p, sz));
+  b.cap = sz;
+  return FMM_OK;
+}
+
+template <typename T>
+int upload(fmm_ctx *c, DBuf &b, const T *h, size_t count) {
+  int rc = ensure(c, b, count * sizeof(T));
+  if (rc) return rc;
+  if (count) HIPCHK(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, c->st));
+  return FMM_OK;
+}
+
+void release(DBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+fmmdev::DevBodies dev_bodies(fmm_ctx *c) {
+  return {c->x.as<double>(), c->y.as<double>(), c->z.as<double>(), c->q.as<double>(),
+          c->pot.as<double>(), c->ax.as<double>(), c->ay.as<double>(), c->az.as<double>()};
+}
+fmmdev::DevCells dev_cells(fmm_ctx *c) {
+  return {c->cx.as<double>(), c->cy.as<double>(),    c->cz.as<double>(),   c->ibody.as<int>(),
+          c->nbody.as<int>(), c->ichild.as<int>(), c->nchild.as<int>(), c->iparent.as<int>()};
+}
+
+int nterm(const fmm_ctx *c) { return c->p.P * (c->p.P + 1) / 2; }
+
+double s_nm(int n, int m) {  // sqrt((n-m)!(n+m)!)
+  double f = 1.0;
+  for (int i = 2; i <= n - m; i++) f *= i;
+  double g = 1.0;
+  for (int i = 2; i <= n + m; i++) g *= i;
+  return std::sqrt(f) * std::sqrt(g);
+}
+
+int check_ready(fmm_ctx *c) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies: call fmm_set_bodies first");
+  HIPCHK(hipSetDevice(c->p.device));
+  return FMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fmm_default_params(fmm_params *p) {
+  if (!p) return;
+  p->P = 10;
+  p->ncrit = 64;
+  p->nspawn = 5000;
+  p->images = 0;
+  p->theta = 0.4;
+  p->eps2 = 0.0;
+  p->precision = FMM_FP64;
+  p->device = 0;
+}
+
+int fmm_create(fmm_ctx **out, const fmm_params *p) {
+  if (!out || !p) return FMM_ERR_ARG;
+  *out = nullptr;
+  fmm_ctx *c = new (std::nothrow) fmm_ctx();
+  if (!c) return FMM_ERR_MEMORY;
+  c->p = *p;
+  int rc = FMM_OK;
+  if (!fmmdev::order_supported(p->P)) {
+    c->err = "unsupported expansion order P=" + std::to_string(p->P) + " (2..20)";
+    rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->ncrit < 1 || p->theta <= 0 || p->nspawn < 1) {
+    c->err = "invalid ncrit/theta/nspawn";
+    rc = FMM_ERR_ARG;
+  } else if (p->images != 0) {
+    c->err = "periodic images are not supported (images must be 0)";
+    rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->precision != FMM_FP64) {
+    c->err = "unsupported precision mode";
+    rc = FMM_ERR_UNSUPPORTED;
+  }
+  if (rc == FMM_OK) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+      c->err = std::string("no HIP device: ") + hipGetErrorString(e);
+      rc = FMM_ERR_DEVICE;
+    } else if (p->device < 0 || p->device >= ndev || p->device >= 64) {
+      c->err = "device ordinal out of range";
+      rc = FMM_ERR_ARG;
+    }
+  }
+  if (rc == FMM_OK) {
+    rc = [&]() -> int {
+      HIPCHK(hipSetDevice(p->device));
+      {
+        std::lock_guard<std::mutex> lk(g_init_mu);
+        if (!g_init_done[p->device]) {
+          HIPCHK(fmmdev::init_constants());
+          g_init_done[p->device] = true;
+        }
+      }
+      HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+      for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+      return FMM_OK;
+    }();
+  }
+  *out = c;  // returned even on failure so fmm_last_error can be read
+  return rc;
+}
+
+void fmm_destroy(fmm_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->p.device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  DBuf *bufs[] = {&c->x,      &c->y,      &c->z,           &c->q,         &c->pot,     &c->ax,
+                  &c->ay,     &c->az,     &c->cx,          &c->cy,        &c->cz,      &c->ibody,
+                  &c->nbody,  &c->ichild, &c->nchild,      &c->iparent,   &c->M,       &c->L,
+                  &c->leaves, &c->internal, &c->level_cells, &c->body_leaf, &c->m2l_ptr, &c->m2l_src,
+                  &c->m2l_tgt, &c->p2p_ptr, &c->p2p_src,     &c->p2p_tgt};
+  for (DBuf *b : bufs) release(*b);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *fmm_last_error(const fmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+namespace {
+void plan_fill(fmm_plan *pl, int n, const double *x, int ncrit, double theta, int nspawn) {
+  pl->n = n;
+  double t0 = now_ms();
+  fmm::get_bounds(n, x, pl->xmin, pl->xmax);
+  double t1 = now_ms();
+  pl->tree = fmm::build_tree(n, x, ncrit, pl->xmin, pl->xmax);
+  double t2 = now_ms();
+  const double xper[3] = {0, 0, 0};
+  pl->lists = fmm::build_lists(pl->tree, theta, nspawn, xper);
+  double t3 = now_ms();
+  pl->t_bounds = t1 - t0;
+  pl->t_tree = t2 - t1;
+  pl->t_lists = t3 - t2;
+}
+
+void plan_stats(const fmm_plan *pl, fmm_stats &s) {
+  s = fmm_stats{};
+  s.n = pl->n;
+  s.ncell = pl->tree.ncell;
+  s.nleaf = (int)pl->tree.leaves.size();
+  s.depth = pl->tree.depth;
+  s.m2l_pairs = pl->lists.nm2l;
+  s.p2p_pairs = pl->lists.np2p;
+  s.p2p_body_pairs = pl->lists.p2p_body_pairs;
+  for (int d = 0; d < 3; d++) {
+    s.xmin[d] = pl->xmin[d];
+    s.xmax[d] = pl->xmax[d];
+  }
+  s.host_bounds_ms = pl->t_bounds;
+  s.host_tree_ms = pl->t_tree;
+  s.host_lists_ms = pl->t_lists;
+}
+
+int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q);
+}  // namespace
+
+int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn) {
+  if (!out) return FMM_ERR_ARG;
+  *out = nullptr;
+  if (n < 0 || (n > 0 && !x) || ncrit < 1 || !(theta > 0) || nspawn < 1) return FMM_ERR_ARG;
+  fmm_plan *pl = new (std::nothrow) fmm_plan();
+  if (!pl) return FMM_ERR_MEMORY;
+  try {
+    plan_fill(pl, n, x, ncrit, theta, nspawn);
+  } catch (const std::bad_alloc &) {
+    delete pl;
+    return FMM_ERR_MEMORY;
+  }
+  *out = pl;
+  return FMM_OK;
+}
+
+void fmm_plan_destroy(fmm_plan *pl) { delete pl; }
+
+int fmm_plan_stats(const fmm_plan *pl, fmm_stats *s) {
+  if (!pl || !s) return FMM_ERR_ARG;
+  plan_stats(pl, *s);
+  return FMM_OK;
+}
+
+namespace {
+void copy_tree(const fmm::Tree &t, int *perm, int *iparent, int *ichild, int *nchild, int *ibody,
+               int *nbody, double *X, double *R) {
+  auto cp = [](int *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(int));
+  };
+  cp(perm, t.perm);
+  cp(iparent, t.iparent);
+  cp(ichild, t.ichild);
+  cp(nchild, t.nchild);
+  cp(ibody, t.ibody);
+  cp(nbody, t.nbody);
+  if (X && !t.X.empty()) std::memcpy(X, t.X.data(), t.X.size() * sizeof(double));
+  if (R && !t.R.empty()) std::memcpy(R, t.R.data(), t.R.size() * sizeof(double));
+}
+void copy_lists(const fmm::Lists &L, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
+  auto cp = [](int *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(int));
+  };
+  cp(m2l_ptr, L.m2l_ptr);
+  cp(m2l_src, L.m2l_src);
+  cp(p2p_ptr, L.p2p_ptr);
+  cp(p2p_src, L.p2p_src);
+}
+}  // namespace
+
+int fmm_plan_tree(const fmm_plan *pl, int *perm, int *iparent, int *ichild, int *nchild, int *ibody,
+                  int *nbody, double *X, double *R) {
+  if (!pl) return FMM_ERR_ARG;
+  copy_tree(pl->tree, perm, iparent, ichild, nchild, ibody, nbody, X, R);
+  return FMM_OK;
+}
+
+int fmm_plan_lists(const fmm_plan *pl, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
+  if (!pl) return FMM_ERR_ARG;
+  copy_lists(pl->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);
+  return FMM_OK;
+}
+
+int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q) {
+  if (!c) return FMM_ERR_ARG;
+  if (n < 0 || (n > 0 && (!x || !q))) FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies: bad n or NULL array");
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  fmm_plan pl;
+  try {
+    plan_fill(&pl, n, x, c->p.ncrit, c->p.theta, c->p.nspawn);
+  } catch (const std::bad_alloc &) {
+    FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while building the tree");
+  }
+  return upload_plan(c, std::move(pl), x, q);
+}
+
+int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const double *q) {
+  if (!c || !pl) return FMM_ERR_ARG;
+  if (pl->n > 0 && (!x || !q)) FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies_plan: NULL array");
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  fmm_plan copy = *pl;
+  return upload_plan(c, std::move(copy), x, q);
+}
+
+namespace {
+int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
+  HIPCHK(hipSetDevice(c->p.device));
+  c->have_bodies = false;
+  const double t3 = now_ms();
+  const int n = pl.n;
+  c->tree = std::move(pl.tree);
+  c->lists = std::move(pl.lists);
+  const fmm::Tree &t = c->tree;
+  const fmm::Lists &L = c->lists;
+  c->n = n;
+  const int nc = t.ncell;
+  // bodies in tree order (SoA)
+  std::vector<double> hx(n), hy(n), hz(n), hq(n);
+  for (int i = 0; i < n; i++) {
+    const int b = t.perm[i];
+    hx[i] = x[3 * (size_t)b];
+    hy[i] = x[3 * (size_t)b + 1];
+    hz[i] = x[3 * (size_t)b + 2];
+    hq[i] = q[b];
+  }
+  std::vector<double> hcx(nc), hcy(nc), hcz(nc);
+  for (int k = 0; k < nc; k++) {
+    hcx[k] = t.X[3 * (size_t)k];
+    hcy[k] = t.X[3 * (size_t)k + 1];
+    hcz[k] = t.X[3 * (size_t)k + 2];
+  }
+  // internal cells per level (M2M), body -> leaf map (L2P)
+  std::vector<int> internal;
+  c->internal_ptr.assign(t.depth + 2, 0);
+  c->level_ptr = t.level_ptr;
+  for (int l = 0; l <= t.depth && nc > 0; l++) {
+    c->internal_ptr[l] = (int)internal.size();
+    for (int k = t.level_ptr[l]; k < t.level_ptr[l + 1]; k++) {
+      const int cell = t.level_cells[k];
+      if (t.nchild[cell] > 0) internal.push_back(cell);
+    }
+  }
+  c->internal_ptr[t.depth + 1] = (int)internal.size();
+  std::vector<int> body_leaf(n);
+  for (int leaf : t.leaves)
+    for (int i = t.ibody[leaf]; i < t.ibody[leaf] + t.nbody[leaf]; i++) body_leaf[i] = leaf;
+  const size_t NT = nterm(c);
+  int rc = FMM_OK;
+#define UP(buf, vec)                                                \
+  if ((rc = upload(c, c->buf, (vec).data(), (vec).size()))) {       \
+    (void)hipStreamSynchronize(c->st); /* staging vectors in use */ \
+    return rc;                                                      \
+  }
+  UP(x, hx) UP(y, hy) UP(z, hz) UP(q, hq)
+  UP(cx, hcx) UP(cy, hcy) UP(cz, hcz)
+  UP(ibody, t.ibody) UP(nbody, t.nbody) UP(ichild, t.ichild) UP(nchild, t.nchild)
+  UP(iparent, t.iparent)
+  UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
+  UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
+  UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
+#undef UP
+  HIPCHK(hipStreamSynchronize(c->st));  // host staging vectors die here
+  if ((rc = ensure(c, c->pot, sizeof(double) * n))) return rc;
+  if ((rc = ensure(c, c->ax, sizeof(double) * n))) return rc;
+  if ((rc = ensure(c, c->ay, sizeof(double) * n))) return rc;
+  if ((rc = ensure(c, c->az, sizeof(double) * n))) return rc;
+  if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
+  if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
+  double t4 = now_ms();
+  pl.tree = fmm::Tree();
+  plan_stats(&pl, c->stats);
+  fmm_stats &s = c->stats;
+  s.n = n;
+  s.ncell = nc;
+  s.nleaf = (int)t.leaves.size();
+  s.depth = t.depth;
+  s.m2l_pairs = L.nm2l;
+  s.p2p_pairs = L.np2p;
+  s.p2p_body_pairs = L.p2p_body_pairs;
+  s.host_upload_ms = t4 - t3;
+  c->have_bodies = true;
+  return FMM_OK;
+}
+}  // namespace
+
+int fmm_set_charges(fmm_ctx *c, const double *q) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!q && c->n > 0) FMM_FAIL(FMM_ERR_ARG, "fmm_set_charges: NULL q");
+  std::vector<double> hq(c->n);
+  for (int i = 0; i < c->n; i++) hq[i] = q[c->tree.perm[i]];
+  if ((rc = upload(c, c->q, hq.data(), hq.size()))) return rc;
+  HIPCHK(hipStreamSynchronize(c->st));
+  return FMM_OK;
+}
+
+int fmm_upward(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->tree.ncell == 0) return FMM_OK;
+  const int P = c->p.P;
+  auto C = dev_cells(c);
+  HIPCHK(fmmdev::launch_p2m(P, dev_bodies(c), C, c->leaves.as<int>(), (int)c->tree.leaves.size(),
+                            c->M.as<double2>(), c->st));
+  for (int l = c->tree.depth; l >= 0; l--) {
+    const int b = c->internal_ptr[l], e = c->internal_ptr[l + 1];
+    if (e > b)
+      HIPCHK(fmmdev::launch_m2m(P, C, c->internal.as<int>() + b, e - b, c->M.as<double2>(), c->st));
+  }
+  return FMM_OK;
+}
+
+int fmm_p2p(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  const size_t bytes = sizeof(double) * c->n;
+  if (c->n > 0) {
+    HIPCHK(hipMemsetAsync(c->pot.p, 0, bytes, c->st));
+    HIPCHK(hipMemsetAsync(c->ax.p, 0, bytes, c->st));
+    HIPCHK(hipMemsetAsync(c->ay.p, 0, bytes, c->st));
+    HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, c->st));
+  }
+  const double3 xper = make_double3(0.0, 0.0, 0.0);
+  HIPCHK(fmmdev::launch_p2p(dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(),
+                            (int)c->lists.p2p_targets.size(), c->p2p_ptr.as<int>(),
+                            c->p2p_src.as<int>(), xper, c->p.eps2, c->st));
+  return FMM_OK;
+}
+
+int fmm_m2l(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->tree.ncell == 0) return FMM_OK;
+  HIPCHK(hipMemsetAsync(c->L.p, 0, sizeof(double2) * nterm(c) * c->tree.ncell, c->st));
+  const double3 xper = make_double3(0.0, 0.0, 0.0);
+  HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->m2l_tgt.as<int>(),
+                            (int)c->lists.m2l_targets.size(), c->m2l_ptr.as<int>(),
+                            c->m2l_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(),
+                            c->st));
+  return FMM_OK;
+}
+
+int fmm_downward(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->tree.ncell == 0) return FMM_OK;
+  const int P = c->p.P;
+  auto C = dev_cells(c);
+  for (int l = 1; l <= c->tree.depth; l++) {
+    const int b = c->level_ptr[l], e = c->level_ptr[l + 1];
+    if (e > b)
+      HIPCHK(fmmdev::launch_l2l(P, C, c->level_cells.as<int>() + b, e - b, c->L.as<double2>(), c->st));
+  }
+  HIPCHK(fmmdev::launch_l2p(P, dev_bodies(c), C, c->body_leaf.as<int>(), c->n, c->L.as<double2>(),
+                            c->st));
+  return FMM_OK;
+}
+
+int fmm_evaluate(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev[0], c->st));
+  if ((rc = fmm_upward(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev[1], c->st));
+  if ((rc = fmm_p2p(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev[2], c->st));
+  if ((rc = fmm_m2l(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev[3], c->st));
+  if ((rc = fmm_downward(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev[4], c->st));
+  HIPCHK(hipEventSynchronize(c->ev[4]));
+  float ms[4];
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
+  float tot;
+  HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[4]));
+  c->stats.dev_upward_ms = ms[0];
+  c->stats.dev_p2p_ms = ms[1];
+  c->stats.dev_m2l_ms = ms[2];
+  c->stats.dev_downward_ms = ms[3];
+  c->stats.dev_total_ms = tot;
+  return FMM_OK;
+}
+
+int fmm_synchronize(fmm_ctx *c) {
+  if (!c) return FMM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->p.device));
+  if (c->st) HIPCHK(hipStreamSynchronize(c->st));
+  return FMM_OK;
+}
+
+int fmm_get_targets(fmm_ctx *c, double *trg, int input_order) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!trg && c->n > 0) FMM_FAIL(FMM_ERR_ARG, "fmm_get_targets: NULL trg");
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int n = c->n;
+  std::vector<double> h(4 * (size_t)n);
+  if (n > 0) {
+    HIPCHK(hipMemcpy(h.data(), c->pot.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data() + n, c->ax.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data() + 2 * (size_t)n, c->ay.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data() + 3 * (size_t)n, c->az.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < n; i++) {
+    const size_t o = input_order ? (size_t)c->tree.perm[i] : (size_t)i;
+    for (int k = 0; k < 4; k++) trg[4 * o + k] = h[(size_t)k * n + i];
+  }
+  return FMM_OK;
+}
+
+int fmm_get_expansions(fmm_ctx *c, double *M, double *L) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int P = c->p.P, NT = nterm(c), nc = c->tree.ncell;
+  std::vector<double> s(NT);
+  for (int n = 0; n < P; n++)
+    for (int m = 0; m <= n; m++) s[n * (n + 1) / 2 + m] = s_nm(n, m);
+  std::vector<double> h(2 * (size_t)NT * nc);
+  if (M && nc) {
+    HIPCHK(hipMemcpy(h.data(), c->M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < (size_t)NT * nc; k++) {
+      M[2 * k] = h[2 * k] * s[k % NT];
+      M[2 * k + 1] = h[2 * k + 1] * s[k % NT];
+    }
+  }
+  if (L && nc) {
+    HIPCHK(hipMemcpy(h.data(), c->L.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < (size_t)NT * nc; k++) {
+      L[2 * k] = h[2 * k] / s[k % NT];
+      L[2 * k + 1] = h[2 * k + 1] / s[k % NT];
+    }
+  }
+  return FMM_OK;
+}
+
+int fmm_get_stats(fmm_ctx *c, fmm_stats *s) {
+  if (!c || !s) return FMM_ERR_ARG;
+  *s = c->stats;
+  return FMM_OK;
+}
+
+int fmm_get_tree(fmm_ctx *c, int *perm, int *iparent, int *ichild, int *nchild, int *ibody,
+                 int *nbody, double *X, double *R) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies");
+  copy_tree(c->tree, perm, iparent, ichild, nchild, ibody, nbody, X, R);
+  return FMM_OK;
+}
+
+int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies");
+  copy_lists(c->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);
+  return FMM_OK;
+}
+
+}  // extern "C"

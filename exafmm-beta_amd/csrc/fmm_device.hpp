@@ -1,0 +1,49 @@
+// fmm_device.hpp -- launchers for the gfx950 Laplace FMM kernels.
+//
+// Device data layout (all in HBM, tree order):
+//   bodies: SoA x, y, z, q and targets pot, ax, ay, az (double)
+//   cells : SoA centres cx, cy, cz; int ibody, nbody, ichild, nchild, iparent
+//   M     : [ncell][NTERM] complex, M~[n,m] = M[n,m] / s(n,m)
+//   L     : [ncell][NTERM] complex, L^[n,m] = L[n,m] * s(n,m)
+// with NTERM = P(P+1)/2, index n(n+1)/2+m, and s(n,m) = sqrt((n-m)!(n+m)!);
+// M and L are exafmm-beta's coefficients (kernels/laplace.h) rescaled so every
+// operator becomes a plain sum over regular / singular solid harmonics.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fmmdev {
+
+struct DevBodies {
+  const double *x, *y, *z, *q;
+  double *pot, *ax, *ay, *az;
+};
+
+struct DevCells {
+  const double *cx, *cy, *cz;
+  const int *ibody, *nbody, *ichild, *nchild, *iparent;
+};
+
+constexpr int kMaxP = 20;
+
+bool order_supported(int P);
+hipError_t init_constants();
+
+// P2M for every leaf in `leaves` (kernels/laplace.h:237-252)
+hipError_t launch_p2m(int P, DevBodies B, DevCells C, const int *leaves, int nleaf, double2 *M,
+                      hipStream_t s);
+// M2M into every parent listed in `cells` from its children (laplace.h:254-290)
+hipError_t launch_m2m(int P, DevCells C, const int *cells, int ncell, double2 *M, hipStream_t s);
+// M2L: L[t] = sum over the CSR list of t (laplace.h:292-322); overwrites L[t]
+hipError_t launch_m2l(int P, DevCells C, const int *targets, int ntarget, const int *ptr,
+                      const int *src, double3 xper, const double2 *M, double2 *L, hipStream_t s);
+// L2L into every cell in `cells` from its parent (laplace.h:324-357); accumulates
+hipError_t launch_l2l(int P, DevCells C, const int *cells, int ncell, double2 *L, hipStream_t s);
+// L2P for every body (laplace.h:359-388); accumulates into the targets
+hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int n,
+                      const double2 *L, hipStream_t s);
+// P2P for every target leaf over its CSR list (laplace.h:150-235); overwrites
+// the targets of each listed leaf
+hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
+                      const int *src, double3 xper, double eps2, hipStream_t s);
+
+}  // namespace fmmdev

@@ -1,0 +1,283 @@
+// fmm_host.cpp -- tree and list construction (see fmm_host.hpp).
+// Compiled with -ffp-contract=off: the octant tests, the box arithmetic and the
+// multipole acceptance criterion must round exactly like the reference's
+// scalar x86 build, or a borderline pair changes list.
+#include "fmm_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace fmm {
+
+void get_bounds(int n, const double *x, double xmin[3], double xmax[3]) {
+  if (n == 0) {
+    for (int d = 0; d < 3; d++) xmin[d] = xmax[d] = 0;
+    return;
+  }
+  for (int d = 0; d < 3; d++) xmin[d] = xmax[d] = x[d];
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      double lo = xmin[d] - 1e-5, hi = xmax[d] + 1e-5, v = x[3 * (size_t)i + d];
+      xmin[d] = v < lo ? v : lo;
+      xmax[d] = v > hi ? v : hi;
+    }
+  }
+}
+
+namespace {
+
+// Octree node during the top-down build (include/build_tree.h:14-20).
+struct Node {
+  int ibody, nbody, nnode;
+  int child[8];  // index into the node pool, -1 if absent
+  double X[3];
+};
+
+struct Builder {
+  const double *x;
+  int ncrit;
+  std::vector<Node> pool;
+
+  int make(int begin, int end, const double *X) {
+    Node nd;
+    nd.ibody = begin;
+    nd.nbody = end - begin;
+    nd.nnode = 1;
+    for (int i = 0; i < 8; i++) nd.child[i] = -1;
+    std::memcpy(nd.X, X, sizeof nd.X);
+    pool.push_back(nd);
+    return (int)pool.size() - 1;
+  }
+
+  static inline int octant(const double *xb, const double *X) {
+    return (xb[0] > X[0]) + ((xb[1] > X[1]) << 1) + ((xb[2] > X[2]) << 2);
+  }
+
+  // buildNodes (include/build_tree.h:77-108).  `bodies` and `buffer` are
+  // permutation arrays; recursion swaps them and a leaf reached with
+  // direction set copies its range back, so the final order lands in the
+  // array passed first at the top level.
+  int build(int *bodies, int *buffer, int begin, int end, const double *X, double R0, int level,
+            bool direction) {
+    if (begin == end) return -1;
+    if (end - begin <= ncrit) {
+      if (direction) std::memcpy(buffer + begin, bodies + begin, sizeof(int) * (end - begin));
+      return make(begin, end, X);
+    }
+    int self = make(begin, end, X);
+    int nb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = begin; i < end; i++) nb[octant(x + 3 * (size_t)bodies[i], X)]++;
+    int off[8], start[8];
+    for (int i = 0, o = begin; i < 8; i++) {
+      off[i] = start[i] = o;
+      o += nb[i];
+    }
+    for (int i = begin; i < end; i++) {
+      int b = bodies[i];
+      buffer[off[octant(x + 3 * (size_t)b, X)]++] = b;
+    }
+    double r = R0 / (1 << (level + 1));
+    int kids[8];
+    for (int i = 0; i < 8; i++) {
+      double Xc[3];
+      for (int d = 0; d < 3; d++) {
+        Xc[d] = X[d];
+        Xc[d] += r * (((i & 1 << d) >> d) * 2 - 1);
+      }
+      kids[i] = build(buffer, bodies, start[i], start[i] + nb[i], Xc, R0, level + 1, !direction);
+    }
+    int nnode = 1;
+    for (int i = 0; i < 8; i++) {
+      pool[self].child[i] = kids[i];
+      if (kids[i] >= 0) nnode += pool[kids[i]].nnode;
+    }
+    pool[self].nnode = nnode;
+    return self;
+  }
+};
+
+// nodes2cells (include/build_tree.h:123-165): a cell's children occupy a
+// contiguous block; each child's descendants follow in child order.
+void nodes2cells(const Builder &B, Tree &t, int node, int C, int CN, double R0, int level,
+                 int iparent) {
+  const Node &nd = B.pool[node];
+  t.iparent[C] = iparent;
+  t.R[C] = R0 / (1 << level);
+  t.level[C] = level;
+  std::memcpy(&t.X[3 * (size_t)C], nd.X, sizeof(double) * 3);
+  t.nbody[C] = nd.nbody;
+  t.ibody[C] = nd.ibody;
+  if (nd.nnode == 1) {
+    t.ichild[C] = 0;
+    t.nchild[C] = 0;
+    t.depth = std::max(t.depth, level);
+    return;
+  }
+  int nchild = 0, kids[8];
+  for (int i = 0; i < 8; i++)
+    if (nd.child[i] >= 0) kids[nchild++] = nd.child[i];
+  int Ci = CN;
+  t.ichild[C] = Ci;
+  t.nchild[C] = nchild;
+  CN += nchild;
+  for (int i = 0; i < nchild; i++) {
+    nodes2cells(B, t, kids[i], Ci, CN, R0, level + 1, C);
+    Ci++;
+    CN += B.pool[kids[i]].nnode - 1;
+  }
+  t.depth = std::max(t.depth, level + 1);
+}
+
+}  // namespace
+
+Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const double xmax[3]) {
+  Tree t;
+  t.n = n;
+  t.perm.resize(n);
+  if (n == 0) {
+    t.level_ptr.assign(1, 0);
+    return t;
+  }
+  // bounds2box (include/build_tree.h:168-180)
+  double X[3], R = 0;
+  for (int d = 0; d < 3; d++) X[d] = (xmax[d] + xmin[d]) / 2;
+  for (int d = 0; d < 3; d++) {
+    R = std::max(X[d] - xmin[d], R);
+    R = std::max(xmax[d] - X[d], R);
+  }
+  R *= 1.00001;
+  std::vector<int> buffer(n);
+  for (int i = 0; i < n; i++) t.perm[i] = i;
+  Builder B{x, ncrit, {}};
+  B.pool.reserve(2 * (size_t)n / std::max(ncrit, 1) + 64);
+  int root = B.build(t.perm.data(), buffer.data(), 0, n, X, R, 0, false);
+  int nc = B.pool[root].nnode;
+  t.ncell = nc;
+  t.iparent.resize(nc);
+  t.ichild.resize(nc);
+  t.nchild.resize(nc);
+  t.ibody.resize(nc);
+  t.nbody.resize(nc);
+  t.level.resize(nc);
+  t.X.resize(3 * (size_t)nc);
+  t.R.resize(nc);
+  nodes2cells(B, t, root, 0, 1, R, 0, 0);
+  // level index and leaf list
+  t.level_ptr.assign(t.depth + 2, 0);
+  for (int c = 0; c < nc; c++) t.level_ptr[t.level[c] + 1]++;
+  for (int l = 0; l <= t.depth; l++) t.level_ptr[l + 1] += t.level_ptr[l];
+  t.level_cells.resize(nc);
+  std::vector<int> fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
+  for (int c = 0; c < nc; c++) t.level_cells[fill[t.level[c]]++] = c;
+  for (int c = 0; c < nc; c++)
+    if (t.nchild[c] == 0) t.leaves.push_back(c);
+  return t;
+}
+
+namespace {
+
+struct Walker {
+  const Tree &t;
+  double theta;
+  int nspawn;
+  double xper[3];
+  std::vector<int> m2l_i, m2l_j, p2p_i, p2p_j;
+
+  void dual(int Ci, int Cj) {
+    // dualTreeTraversal (include/traversal.h:117-177)
+    double dX[3];
+    for (int d = 0; d < 3; d++) dX[d] = t.X[3 * (size_t)Ci + d] - t.X[3 * (size_t)Cj + d] - xper[d];
+    double RT2 = (dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2]) * theta * theta;
+    double Rs = t.R[Ci] + t.R[Cj];
+    if (RT2 > Rs * Rs * (1 - 1e-3)) {
+      m2l_i.push_back(Ci);
+      m2l_j.push_back(Cj);
+    } else if (t.nchild[Ci] == 0 && t.nchild[Cj] == 0) {
+      if (t.nbody[Cj] == 0) {
+        m2l_i.push_back(Ci);
+        m2l_j.push_back(Cj);
+      } else {
+        p2p_i.push_back(Ci);
+        p2p_j.push_back(Cj);
+      }
+    } else {
+      split(Ci, Cj);
+    }
+  }
+
+  // TraverseRange::operator() executed in task order (traversal.h:196-229)
+  void range(int ib, int ie, int jb, int je) {
+    if (ie - ib == 1 || je - jb == 1) {
+      if (ib == jb) {
+        dual(ib, jb);
+      } else {
+        for (int i = ib; i < ie; i++)
+          for (int j = jb; j < je; j++) dual(i, j);
+      }
+    } else {
+      int im = ib + (ie - ib) / 2, jm = jb + (je - jb) / 2;
+      range(ib, im, jb, jm);
+      range(im, ie, jm, je);
+      range(ib, im, jm, je);
+      range(im, ie, jb, jm);
+    }
+  }
+
+  // splitCell (include/traversal.h:90-114)
+  void split(int Ci, int Cj) {
+    if (t.nchild[Cj] == 0) {
+      for (int ci = t.ichild[Ci]; ci < t.ichild[Ci] + t.nchild[Ci]; ci++) dual(ci, Cj);
+    } else if (t.nchild[Ci] == 0) {
+      for (int cj = t.ichild[Cj]; cj < t.ichild[Cj] + t.nchild[Cj]; cj++) dual(Ci, cj);
+    } else if (t.nbody[Ci] + t.nbody[Cj] >= nspawn || Ci == Cj) {
+      range(t.ichild[Ci], t.ichild[Ci] + t.nchild[Ci], t.ichild[Cj], t.ichild[Cj] + t.nchild[Cj]);
+    } else if (t.R[Ci] >= t.R[Cj]) {
+      for (int ci = t.ichild[Ci]; ci < t.ichild[Ci] + t.nchild[Ci]; ci++) dual(ci, Cj);
+    } else {
+      for (int cj = t.ichild[Cj]; cj < t.ichild[Cj] + t.nchild[Cj]; cj++) dual(Ci, cj);
+    }
+  }
+};
+
+// Stable counting sort of (target, source) pairs by target: each target's
+// list keeps the traversal's visiting order.
+void to_csr(int ncell, const std::vector<int> &ti, const std::vector<int> &sj,
+            std::vector<int> &ptr, std::vector<int> &src, std::vector<int> &targets) {
+  ptr.assign(ncell + 1, 0);
+  for (int c : ti) ptr[c + 1]++;
+  for (int c = 0; c < ncell; c++) ptr[c + 1] += ptr[c];
+  src.resize(ti.size());
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (size_t k = 0; k < ti.size(); k++) src[fill[ti[k]]++] = sj[k];
+  targets.clear();
+  for (int c = 0; c < ncell; c++)
+    if (ptr[c + 1] > ptr[c]) targets.push_back(c);
+}
+
+}  // namespace
+
+Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]) {
+  Lists L;
+  L.m2l_ptr.assign(t.ncell + 1, 0);
+  L.p2p_ptr.assign(t.ncell + 1, 0);
+  if (t.ncell == 0) return L;
+  Walker w{t, theta, nspawn, {0, 0, 0}, {}, {}, {}, {}};
+  if (xper) std::memcpy(w.xper, xper, sizeof w.xper);
+  w.m2l_i.reserve(64 * (size_t)t.ncell);
+  w.m2l_j.reserve(64 * (size_t)t.ncell);
+  w.p2p_i.reserve(32 * (size_t)t.leaves.size());
+  w.p2p_j.reserve(32 * (size_t)t.leaves.size());
+  w.dual(0, 0);
+  L.nm2l = (int64_t)w.m2l_i.size();
+  L.np2p = (int64_t)w.p2p_i.size();
+  to_csr(t.ncell, w.m2l_i, w.m2l_j, L.m2l_ptr, L.m2l_src, L.m2l_targets);
+  to_csr(t.ncell, w.p2p_i, w.p2p_j, L.p2p_ptr, L.p2p_src, L.p2p_targets);
+  int64_t bp = 0;
+  for (size_t k = 0; k < w.p2p_i.size(); k++)
+    bp += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
+  L.p2p_body_pairs = bp;
+  return L;
+}
+
+}  // namespace fmm

@@ -1,0 +1,50 @@
+// fmm_host.hpp -- host-side tree and interaction-list construction for the
+// MI355X Laplace FMM evaluator.  These produce exactly the tree and the pair
+// lists of exafmm-beta (bit-identical decisions), so the device results can be
+// compared with the reference FMM at the same P, theta and ncrit.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace fmm {
+
+// Flattened octree in the reference's cell order (nodes2cells,
+// include/build_tree.h:123-165): children of a cell are contiguous, bodies of
+// a cell are a contiguous range in tree order.
+struct Tree {
+  int n = 0;                 // bodies
+  int ncell = 0;
+  int depth = 0;             // deepest level (root = 0)
+  std::vector<int> perm;     // tree-order position -> input index
+  std::vector<int> iparent, ichild, nchild, ibody, nbody, level;
+  std::vector<double> X;     // 3 * ncell, cell centres
+  std::vector<double> R;     // cell radii
+  // Cells grouped by level (level_ptr[l] .. level_ptr[l+1] in level_cells)
+  std::vector<int> level_ptr, level_cells;
+  std::vector<int> leaves;   // leaf cells in cell order
+};
+
+// Interaction lists in CSR form per target cell, each list in the order the
+// reference's dual tree traversal visits the pairs.
+struct Lists {
+  int64_t nm2l = 0, np2p = 0;
+  std::vector<int> m2l_ptr, m2l_src;   // ncell+1, nm2l
+  std::vector<int> p2p_ptr, p2p_src;   // ncell+1, np2p
+  std::vector<int> m2l_targets;        // cells with a non-empty M2L list
+  std::vector<int> p2p_targets;        // leaves with a non-empty P2P list
+  int64_t p2p_body_pairs = 0;          // sum over P2P pairs of Ni*Nj
+};
+
+// BoundBox::getBounds (include/bound_box.h:11-25): sequential scan that widens
+// the box by 1e-5 per body, as the reference does.
+void get_bounds(int n, const double *x, double xmin[3], double xmax[3]);
+
+// BuildTree::buildTree (include/build_tree.h:168-209).  x is (n,3) in input
+// order.  Returns the tree; perm gives the body order.
+Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const double xmax[3]);
+
+// Traversal::traverse with images = 0 (include/traversal.h:117-177, 90-114,
+// 180-230, 326-352), recording pairs instead of evaluating them.
+Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]);
+
+}  // namespace fmm

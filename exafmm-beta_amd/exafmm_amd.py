@@ -1,0 +1,275 @@
+"""Python binding of libexafmm_amd.so (C ABI: include/exafmm_amd.h).
+
+Mirrors the call surface of exafmm-beta's Laplace path (examples/fmm.cxx:61-77):
+``FMM(P, theta, ncrit, nspawn)`` holds what Kernel/Traversal/BuildTree hold,
+``set_bodies`` does getBounds + buildTree + the traversal's list generation,
+``upward``/``m2l``/``p2p``/``downward`` are the device phases and ``targets``
+returns Body::TRG (potential, gradient).  There is no CPU fallback: without
+the built library (or without a HIP device) every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libexafmm_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# public symbols of include/exafmm_amd.h
+SYMBOLS = [
+    "fmm_default_params", "fmm_create", "fmm_destroy", "fmm_last_error", "fmm_set_bodies",
+    "fmm_set_charges", "fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
+    "fmm_synchronize", "fmm_get_targets", "fmm_get_expansions", "fmm_get_stats", "fmm_get_tree",
+    "fmm_get_lists", "fmm_set_bodies_plan", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
+    "fmm_plan_tree", "fmm_plan_lists",
+]
+
+FMM_FP64 = 0
+
+
+class FMMError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_int), ("ncrit", ctypes.c_int), ("nspawn", ctypes.c_int),
+                ("images", ctypes.c_int), ("theta", ctypes.c_double), ("eps2", ctypes.c_double),
+                ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ncell", ctypes.c_int), ("nleaf", ctypes.c_int),
+                ("depth", ctypes.c_int), ("m2l_pairs", ctypes.c_int64),
+                ("p2p_pairs", ctypes.c_int64), ("p2p_body_pairs", ctypes.c_int64),
+                ("xmin", ctypes.c_double * 3), ("xmax", ctypes.c_double * 3),
+                ("host_bounds_ms", ctypes.c_double), ("host_tree_ms", ctypes.c_double),
+                ("host_lists_ms", ctypes.c_double), ("host_upload_ms", ctypes.c_double),
+                ("dev_upward_ms", ctypes.c_double), ("dev_p2p_ms", ctypes.c_double),
+                ("dev_m2l_ms", ctypes.c_double), ("dev_downward_ms", ctypes.c_double),
+                ("dev_total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name in ("xmin", "xmax") else v
+        return d
+
+
+_lib = None
+
+
+def load():
+    """Load the library (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FMMError("libexafmm_amd.so is not built (%s): run __graft_entry__.build()"
+                           % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.fmm_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.fmm_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params)]
+        L.fmm_destroy.argtypes = [vp]
+        L.fmm_last_error.restype = ctypes.c_char_p
+        L.fmm_last_error.argtypes = [vp]
+        L.fmm_set_bodies.argtypes = [vp, ctypes.c_int, _dp, _dp]
+        L.fmm_set_charges.argtypes = [vp, _dp]
+        for f in ("fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
+                  "fmm_synchronize"):
+            getattr(L, f).argtypes = [vp]
+        L.fmm_get_targets.argtypes = [vp, _dp, ctypes.c_int]
+        L.fmm_get_expansions.argtypes = [vp, _dp, _dp]
+        L.fmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.fmm_get_tree.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
+        L.fmm_get_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
+        L.fmm_set_bodies_plan.argtypes = [vp, vp, _dp, _dp]
+        L.fmm_plan_build.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_int]
+        L.fmm_plan_destroy.argtypes = [vp]
+        L.fmm_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.fmm_plan_tree.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
+        L.fmm_plan_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class FMM:
+    """One evaluator context (HIP stream + device arena)."""
+
+    def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=5000, eps2=0.0, device=0,
+                 precision=FMM_FP64):
+        L = load()
+        p = Params()
+        L.fmm_default_params(ctypes.byref(p))
+        p.P, p.theta, p.ncrit, p.nspawn, p.eps2 = P, theta, ncrit, nspawn, eps2
+        p.device, p.precision = device, precision
+        self.P = P
+        self.nterm = P * (P + 1) // 2
+        self._h = ctypes.c_void_p()
+        rc = L.fmm_create(ctypes.byref(self._h), ctypes.byref(p))
+        if rc != 0:
+            msg = L.fmm_last_error(self._h).decode() if self._h else "fmm_create failed"
+            if self._h:
+                L.fmm_destroy(self._h)
+                self._h = ctypes.c_void_p()
+            raise FMMError("fmm_create: %s (rc=%d)" % (msg, rc))
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            load().fmm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise FMMError("%s: %s (rc=%d)" % (what, load().fmm_last_error(self._h).decode(), rc))
+
+    def set_bodies_plan(self, plan, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        self.n = len(plan.x)
+        self._check(load().fmm_set_bodies_plan(self._h, plan._h, _ptr(plan.x), _ptr(q)),
+                    "fmm_set_bodies_plan")
+
+    def set_bodies(self, x, q):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if len(q) != len(x):
+            raise ValueError("x and q lengths differ")
+        self.n = len(x)
+        self._check(load().fmm_set_bodies(self._h, self.n, _ptr(x), _ptr(q)), "fmm_set_bodies")
+
+    def set_charges(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        self._check(load().fmm_set_charges(self._h, _ptr(q)), "fmm_set_charges")
+
+    def upward(self):
+        self._check(load().fmm_upward(self._h), "fmm_upward")
+
+    def p2p(self):
+        self._check(load().fmm_p2p(self._h), "fmm_p2p")
+
+    def m2l(self):
+        self._check(load().fmm_m2l(self._h), "fmm_m2l")
+
+    def downward(self):
+        self._check(load().fmm_downward(self._h), "fmm_downward")
+
+    def evaluate(self):
+        self._check(load().fmm_evaluate(self._h), "fmm_evaluate")
+
+    def synchronize(self):
+        self._check(load().fmm_synchronize(self._h), "fmm_synchronize")
+
+    def targets(self, input_order=True):
+        trg = np.zeros((self.n, 4))
+        self._check(load().fmm_get_targets(self._h, _ptr(trg), 1 if input_order else 0),
+                    "fmm_get_targets")
+        return trg
+
+    def stats(self):
+        s = Stats()
+        self._check(load().fmm_get_stats(self._h, ctypes.byref(s)), "fmm_get_stats")
+        return s.as_dict()
+
+    def expansions(self):
+        nc = self.stats()["ncell"]
+        M = np.zeros((nc, self.nterm), np.complex128)
+        Lc = np.zeros((nc, self.nterm), np.complex128)
+        self._check(load().fmm_get_expansions(self._h, _ptr(M.view(np.float64)),
+                                              _ptr(Lc.view(np.float64))), "fmm_get_expansions")
+        return M, Lc
+
+    def tree(self):
+        s = self.stats()
+        nc, n = s["ncell"], s["n"]
+        a = {k: np.zeros(nc, np.int32) for k in ("iparent", "ichild", "nchild", "ibody", "nbody")}
+        perm = np.zeros(n, np.int32)
+        X = np.zeros((nc, 3))
+        R = np.zeros(nc)
+        self._check(load().fmm_get_tree(self._h, _ptr(perm, _ip), _ptr(a["iparent"], _ip),
+                                        _ptr(a["ichild"], _ip), _ptr(a["nchild"], _ip),
+                                        _ptr(a["ibody"], _ip), _ptr(a["nbody"], _ip), _ptr(X),
+                                        _ptr(R)), "fmm_get_tree")
+        a.update(perm=perm, X=X, R=R, depth=s["depth"])
+        return a
+
+    def lists(self):
+        s = self.stats()
+        nc = s["ncell"]
+        m2l_ptr = np.zeros(nc + 1, np.int32)
+        p2p_ptr = np.zeros(nc + 1, np.int32)
+        m2l_src = np.zeros(max(s["m2l_pairs"], 1), np.int32)
+        p2p_src = np.zeros(max(s["p2p_pairs"], 1), np.int32)
+        self._check(load().fmm_get_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip),
+                                         _ptr(p2p_ptr, _ip), _ptr(p2p_src, _ip)), "fmm_get_lists")
+        return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]
+
+
+class Plan:
+    """Host-only tree + interaction lists (fmm_plan_*): no device needed."""
+
+    def __init__(self, x, ncrit=64, theta=0.4, nspawn=5000):
+        L = load()
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        self.x = x
+        self._h = ctypes.c_void_p()
+        rc = L.fmm_plan_build(ctypes.byref(self._h), len(x), _ptr(x), ncrit, theta, nspawn)
+        if rc != 0:
+            raise FMMError("fmm_plan_build failed (rc=%d)" % rc)
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().fmm_plan_destroy(self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        load().fmm_plan_stats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def tree(self):
+        s = self.stats()
+        nc, n = s["ncell"], s["n"]
+        a = {k: np.zeros(nc, np.int32) for k in ("iparent", "ichild", "nchild", "ibody", "nbody")}
+        perm = np.zeros(n, np.int32)
+        X = np.zeros((nc, 3))
+        R = np.zeros(nc)
+        load().fmm_plan_tree(self._h, _ptr(perm, _ip), _ptr(a["iparent"], _ip),
+                             _ptr(a["ichild"], _ip), _ptr(a["nchild"], _ip), _ptr(a["ibody"], _ip),
+                             _ptr(a["nbody"], _ip), _ptr(X), _ptr(R))
+        a.update(perm=perm, X=X, R=R, depth=s["depth"])
+        return a
+
+    def lists(self):
+        s = self.stats()
+        nc = s["ncell"]
+        m2l_ptr = np.zeros(nc + 1, np.int32)
+        p2p_ptr = np.zeros(nc + 1, np.int32)
+        m2l_src = np.zeros(max(s["m2l_pairs"], 1), np.int32)
+        p2p_src = np.zeros(max(s["p2p_pairs"], 1), np.int32)
+        load().fmm_plan_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip), _ptr(p2p_ptr, _ip),
+                              _ptr(p2p_src, _ip))
+        return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]

@@ -1,0 +1,125 @@
+/*
+ * exafmm_amd.h -- C ABI of the MI355X Laplace FMM evaluator (libexafmm_amd.so).
+ *
+ * Plain pointers and sizes only.  Bodies are passed as x[3*i+d] (the layout of
+ * exafmm-beta's Body::X), q[i] (Body::SRC); targets come back as trg[4*i+c]
+ * with c = potential, d/dx, d/dy, d/dz (Body::TRG, include/types.h:72-84).
+ * Expansions use exafmm-beta's coefficients: [ncell][NTERM] complex stored as
+ * interleaved doubles, NTERM = P(P+1)/2, index n(n+1)/2+m
+ * (kernels/laplace.h:246-247).
+ *
+ * Every entry point returns 0 on success and a negative fmm_status otherwise;
+ * fmm_last_error() then holds a message.  No exception crosses the ABI.
+ * Calls on one context are serialised by the caller; separate contexts may be
+ * used from separate host threads.
+ */
+#ifndef EXAFMM_AMD_H
+#define EXAFMM_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fmm_ctx fmm_ctx;
+
+typedef enum {
+  FMM_OK = 0,
+  FMM_ERR_ARG = -1,      /* invalid argument (size, order, NULL pointer) */
+  FMM_ERR_STATE = -2,    /* call out of order (e.g. evaluate before bodies) */
+  FMM_ERR_DEVICE = -3,   /* HIP runtime error */
+  FMM_ERR_MEMORY = -4,   /* device or host allocation failed */
+  FMM_ERR_UNSUPPORTED = -5
+} fmm_status;
+
+typedef enum { FMM_FP64 = 0 } fmm_precision;
+
+/* The knobs of exafmm-beta's Kernel / Traversal / BuildTree constructors and
+ * Args defaults (include/args.h:188-205): P=10, theta=0.4, ncrit=64,
+ * nspawn=5000, images=0, eps2=0. */
+typedef struct {
+  int P;          /* expansion order, 2..20 */
+  int ncrit;      /* max bodies per leaf (BuildTree(ncrit)) */
+  int nspawn;     /* Traversal nspawn: both-sides split threshold */
+  int images;     /* periodic image levels; only 0 is supported */
+  double theta;   /* multipole acceptance parameter */
+  double eps2;    /* softening added to r^2 in P2P */
+  int precision;  /* fmm_precision */
+  int device;     /* HIP device ordinal */
+} fmm_params;
+
+typedef struct {
+  int n, ncell, nleaf, depth;
+  int64_t m2l_pairs, p2p_pairs, p2p_body_pairs;
+  double xmin[3], xmax[3];
+  /* host wall times of the last fmm_set_bodies (ms) */
+  double host_bounds_ms, host_tree_ms, host_lists_ms, host_upload_ms;
+  /* device times of the last evaluation (ms, HIP events on the ctx stream) */
+  double dev_upward_ms, dev_p2p_ms, dev_m2l_ms, dev_downward_ms, dev_total_ms;
+} fmm_stats;
+
+/* Fill p with the reference defaults (include/args.h:188-205). */
+void fmm_default_params(fmm_params *p);
+
+/* ---- host-only plan (no device needed) ----
+ * BoundBox::getBounds + BuildTree::buildTree + the pair lists of
+ * Traversal::traverse (include/bound_box.h:11-25, include/build_tree.h:186-209,
+ * include/traversal.h:326-352), built on the host.  fmm_set_bodies builds one
+ * internally; a plan can also be built ahead (e.g. on another host thread). */
+typedef struct fmm_plan fmm_plan;
+int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn);
+void fmm_plan_destroy(fmm_plan *pl);
+/* fills n, ncell, nleaf, depth, pair counts, bounds and host times */
+int fmm_plan_stats(const fmm_plan *pl, fmm_stats *s);
+int fmm_plan_tree(const fmm_plan *pl, int *perm, int *iparent, int *ichild, int *nchild,
+                  int *ibody, int *nbody, double *X, double *R);
+int fmm_plan_lists(const fmm_plan *pl, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src);
+
+/* Replaces `Kernel kernel(P, eps2, wavek)` + `Traversal(kernel, theta,
+ * nspawn, images, path)` + `BuildTree(ncrit)` construction
+ * (examples/fmm.cxx:21-26): one context owns a HIP stream and device memory. */
+int fmm_create(fmm_ctx **out, const fmm_params *p);
+void fmm_destroy(fmm_ctx *c);
+const char *fmm_last_error(const fmm_ctx *c);
+
+/* Replaces BoundBox::getBounds + BuildTree::buildTree (include/bound_box.h:11-25,
+ * include/build_tree.h:186-209) and the list part of Traversal::traverse
+ * (include/traversal.h:326-352): host bounds, octree and interaction lists,
+ * then upload of bodies (in tree order), cells and lists.  x is (n,3) in the
+ * caller's order. */
+int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q);
+/* Same, with a plan built by fmm_plan_build for these x (the context copies
+ * what it needs; the plan stays owned by the caller). */
+int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const double *q);
+
+/* Re-upload charges only (same positions, same tree and lists). */
+int fmm_set_charges(fmm_ctx *c, const double *q);
+
+/* Device phases on the context stream (asynchronous; fmm_synchronize or a
+ * getter waits).  They replace UpDownPass::upwardPass (include/up_down_pass.h:49-60),
+ * the Kernel::M2L / Kernel::P2P calls made by Traversal::dualTreeTraversal
+ * (include/traversal.h:117-177) and UpDownPass::downwardPass (:63-75). */
+int fmm_upward(fmm_ctx *c);
+int fmm_p2p(fmm_ctx *c);
+int fmm_m2l(fmm_ctx *c);
+int fmm_downward(fmm_ctx *c);
+/* upward + p2p + m2l + downward, with per-phase device timing */
+int fmm_evaluate(fmm_ctx *c);
+int fmm_synchronize(fmm_ctx *c);
+
+/* Targets (n x 4). input_order != 0 returns them in the order of the x passed
+ * to fmm_set_bodies, else in tree order (exafmm-beta leaves bodies sorted). */
+int fmm_get_targets(fmm_ctx *c, double *trg, int input_order);
+/* M and L of every cell (may be NULL), reference normalisation. */
+int fmm_get_expansions(fmm_ctx *c, double *M, double *L);
+
+/* Tree and lists as built by fmm_set_bodies (for parity checks).  Any pointer
+ * may be NULL.  perm[i] = input index of the i-th body in tree order. */
+int fmm_get_stats(fmm_ctx *c, fmm_stats *s);
+int fmm_get_tree(fmm_ctx *c, int *perm, int *iparent, int *ichild, int *nchild, int *ibody,
+                 int *nbody, double *X, double *R);
+int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
