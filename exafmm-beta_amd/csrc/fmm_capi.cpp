@@ -39,6 +39,8 @@ struct fmm_ctx {
   std::string err;
   hipStream_t st = nullptr;
   hipEvent_t ev[5] = {};
+  hipEvent_t kev[4] = {};  // M2L kernel begin/end, P2P kernel begin/end
+  bool timing = false;     // set by fmm_evaluate
   fmm::Tree tree;
   fmm::Lists lists;
   int n = 0;
@@ -191,6 +193,7 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
       }
       HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
       for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+      for (auto &e : c->kev) HIPCHK(hipEventCreate(&e));
       return FMM_OK;
     }();
   }
@@ -209,6 +212,8 @@ void fmm_destroy(fmm_ctx *c) {
                   &c->m2l_tgt, &c->p2p_ptr, &c->p2p_src,     &c->p2p_tgt};
   for (DBuf *b : bufs) release(*b);
   for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->kev)
     if (e) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -252,6 +257,11 @@ void plan_stats(const fmm_plan *pl, fmm_stats &s) {
 
 int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q);
 }  // namespace
+
+int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q) {
+  if (n < 0 || (n > 0 && (!x || !q || !dist))) return FMM_ERR_ARG;
+  return fmm::make_bodies(n, dist, seed, x, q) ? FMM_OK : FMM_ERR_ARG;
+}
 
 int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn) {
   if (!out) return FMM_ERR_ARG;
@@ -456,9 +466,11 @@ int fmm_p2p(fmm_ctx *c) {
     HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, c->st));
   }
   const double3 xper = make_double3(0.0, 0.0, 0.0);
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[2], c->st));
   HIPCHK(fmmdev::launch_p2p(dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(),
                             (int)c->lists.p2p_targets.size(), c->p2p_ptr.as<int>(),
                             c->p2p_src.as<int>(), xper, c->p.eps2, c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[3], c->st));
   return FMM_OK;
 }
 
@@ -468,10 +480,12 @@ int fmm_m2l(fmm_ctx *c) {
   if (c->tree.ncell == 0) return FMM_OK;
   HIPCHK(hipMemsetAsync(c->L.p, 0, sizeof(double2) * nterm(c) * c->tree.ncell, c->st));
   const double3 xper = make_double3(0.0, 0.0, 0.0);
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->m2l_tgt.as<int>(),
                             (int)c->lists.m2l_targets.size(), c->m2l_ptr.as<int>(),
                             c->m2l_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(),
                             c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[1], c->st));
   return FMM_OK;
 }
 
@@ -494,6 +508,8 @@ int fmm_downward(fmm_ctx *c) {
 int fmm_evaluate(fmm_ctx *c) {
   int rc = check_ready(c);
   if (rc) return rc;
+  c->timing = true;
+  rc = [&]() -> int {
   HIPCHK(hipEventRecord(c->ev[0], c->st));
   if ((rc = fmm_upward(c))) return rc;
   HIPCHK(hipEventRecord(c->ev[1], c->st));
@@ -513,7 +529,53 @@ int fmm_evaluate(fmm_ctx *c) {
   c->stats.dev_m2l_ms = ms[2];
   c->stats.dev_downward_ms = ms[3];
   c->stats.dev_total_ms = tot;
+  float km = 0.f, kp = 0.f;
+  if (c->tree.ncell > 0) {
+    HIPCHK(hipEventElapsedTime(&km, c->kev[0], c->kev[1]));
+    HIPCHK(hipEventElapsedTime(&kp, c->kev[2], c->kev[3]));
+  }
+  c->stats.dev_m2l_kernel_ms = km;
+  c->stats.dev_p2p_kernel_ms = kp;
   return FMM_OK;
+  }();
+  c->timing = false;
+  return rc;
+}
+
+int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (nt < 0 || (nt > 0 && (!xt || !trg))) FMM_FAIL(FMM_ERR_ARG, "fmm_direct: bad arguments");
+  if (nt == 0) return FMM_OK;
+  std::vector<double> h(3 * (size_t)nt);
+  for (int i = 0; i < nt; i++)
+    for (int d = 0; d < 3; d++) h[(size_t)d * nt + i] = xt[3 * (size_t)i + d];
+  DBuf tb, ob;
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(c->st);
+    release(tb);
+    release(ob);
+    return r;
+  };
+  if ((rc = upload(c, tb, h.data(), h.size()))) return done(rc);
+  if ((rc = ensure(c, ob, sizeof(double) * 4 * nt))) return done(rc);
+  const double *tx = tb.as<double>();
+  hipError_t e = fmmdev::launch_direct(dev_bodies(c), c->n, tx, tx + nt, tx + 2 * (size_t)nt, nt,
+                                       ob.as<double>(), c->p.eps2, c->st);
+  if (e != hipSuccess) {
+    c->err = std::string("direct kernel launch failed: ") + hipGetErrorString(e);
+    return done(FMM_ERR_DEVICE);
+  }
+  std::vector<double> o(4 * (size_t)nt);
+  e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(o.data(), ob.p, o.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    c->err = std::string("fmm_direct: ") + hipGetErrorString(e);
+    return done(FMM_ERR_DEVICE);
+  }
+  for (int i = 0; i < nt; i++)
+    for (int k = 0; k < 4; k++) trg[4 * (size_t)i + k] = o[(size_t)k * nt + i];
+  return done(FMM_OK);
 }
 
 int fmm_synchronize(fmm_ctx *c) {

@@ -46,4 +46,9 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
 hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
                       const int *src, double3 xper, double eps2, hipStream_t s);
 
+// Direct sum (include/traversal.h:355-377) of all n sources at nt targets
+// (SoA txyz); writes tpot/tax/tay/taz.
+hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz,
+                         int nt, double *out, double eps2, hipStream_t s);
+
 }  // namespace fmmdev

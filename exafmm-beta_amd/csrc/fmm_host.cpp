@@ -5,8 +5,11 @@
 #include "fmm_host.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 
 namespace fmm {
 
@@ -278,6 +281,101 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3])
     bp += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
   L.p2p_body_pairs = bp;
   return L;
+}
+
+namespace {
+void reference_charges(int n, int seed, double *q) {  // initSource, dataset.h:142-159
+  srand48(seed);
+  double average = 0;
+  for (int i = 0; i < n; i++) {
+    q[i] = drand48() - .5;
+    average += q[i];
+  }
+  average /= n;
+  for (int i = 0; i < n; i++) q[i] -= average;
+}
+}  // namespace
+
+bool make_bodies(int n, const char *dist, int seed, double *x, double *q) {
+  if (n <= 0) return true;
+  const std::string d(dist ? dist : "");
+  if (d == "cube") {
+    srand48(seed);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) x[3 * (size_t)i + k] = drand48() * 2 * M_PI - M_PI;
+    reference_charges(n, seed, q);
+  } else if (d == "sphere") {
+    srand48(seed);
+    for (int i = 0; i < n; i++) {
+      double *b = x + 3 * (size_t)i;
+      for (int k = 0; k < 3; k++) b[k] = drand48() * 2 - 1;
+      double r = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      for (int k = 0; k < 3; k++) b[k] *= M_PI / r;
+    }
+    reference_charges(n, seed, q);
+  } else if (d == "octant") {
+    srand48(seed);
+    for (int i = 0; i < n; i++) {
+      double th = drand48() * M_PI * 0.5, ph = drand48() * M_PI * 0.5;
+      double *b = x + 3 * (size_t)i;
+      b[0] = 2 * M_PI * std::sin(th) * std::cos(ph) - M_PI;
+      b[1] = 2 * M_PI * std::sin(th) * std::sin(ph) - M_PI;
+      b[2] = 2 * M_PI * std::cos(th) - M_PI;
+    }
+    reference_charges(n, seed, q);
+  } else if (d == "plummer" || d == "plummer1") {
+    const bool unit = d == "plummer1";
+    const double rmax = unit ? 10.0 : 100.0, scale = unit ? 1.0 : 3.0 * M_PI / 16.0;
+    srand48(seed);
+    int i = 0;
+    while (i < n) {
+      double X1 = drand48(), X2 = drand48(), X3 = drand48();
+      double R = 1.0 / std::sqrt((std::pow(X1, -2.0 / 3.0) - 1.0));
+      if (R < rmax) {
+        double Z = (1.0 - 2.0 * X2) * R;
+        double X = std::sqrt(R * R - Z * Z) * std::cos(2.0 * M_PI * X3);
+        double Y = std::sqrt(R * R - Z * Z) * std::sin(2.0 * M_PI * X3);
+        X *= scale;
+        Y *= scale;
+        Z *= scale;
+        x[3 * (size_t)i] = X;
+        x[3 * (size_t)i + 1] = Y;
+        x[3 * (size_t)i + 2] = Z;
+        i++;
+      }
+    }
+    if (unit)
+      for (int k = 0; k < n; k++) q[k] = 1.0 / n;
+    else
+      reference_charges(n, seed, q);
+  } else if (d == "lattice") {
+    int nx = int(std::pow(n, 1. / 3));
+    int k = 0;
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < nx; iy++)
+        for (int iz = 0; iz < nx; iz++, k++) {
+          x[3 * (size_t)k] = (ix / double(nx - 1)) * 2 - 1;
+          x[3 * (size_t)k + 1] = (iy / double(nx - 1)) * 2 - 1;
+          x[3 * (size_t)k + 2] = (iz / double(nx - 1)) * 2 - 1;
+        }
+    for (; k < n; k++) x[3 * (size_t)k] = x[3 * (size_t)k + 1] = x[3 * (size_t)k + 2] = 0;
+    reference_charges(n, seed, q);
+  } else if (d == "cube1") {
+    srand48(seed);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) x[3 * (size_t)i + k] = 2 * drand48() - 1;
+    srand48(seed);
+    double average = 0;
+    for (int i = 0; i < n; i++) {
+      q[i] = 2 * drand48() - 1;
+      average += q[i];
+    }
+    average /= n;
+    for (int i = 0; i < n; i++) q[i] -= average;
+  } else {
+    return false;
+  }
+  return true;
 }
 
 }  // namespace fmm

@@ -24,7 +24,8 @@ SYMBOLS = [
     "fmm_set_charges", "fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
     "fmm_synchronize", "fmm_get_targets", "fmm_get_expansions", "fmm_get_stats", "fmm_get_tree",
     "fmm_get_lists", "fmm_set_bodies_plan", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
-    "fmm_plan_tree", "fmm_plan_lists",
+    "fmm_plan_tree", "fmm_plan_lists", "fmm_make_bodies",
+    "fmm_direct",
 ]
 
 FMM_FP64 = 0
@@ -49,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("host_lists_ms", ctypes.c_double), ("host_upload_ms", ctypes.c_double),
                 ("dev_upward_ms", ctypes.c_double), ("dev_p2p_ms", ctypes.c_double),
                 ("dev_m2l_ms", ctypes.c_double), ("dev_downward_ms", ctypes.c_double),
-                ("dev_total_ms", ctypes.c_double)]
+                ("dev_total_ms", ctypes.c_double), ("dev_m2l_kernel_ms", ctypes.c_double),
+                ("dev_p2p_kernel_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {}
@@ -93,8 +95,20 @@ def load():
         L.fmm_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.fmm_plan_tree.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
         L.fmm_plan_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
+        L.fmm_make_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        L.fmm_direct.argtypes = [vp, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
+
+
+def make_bodies(n, dist="cube", seed=0):
+    """Dataset::initBodies equivalent (host only): returns x (n,3), q (n,)."""
+    x = np.zeros((n, 3))
+    q = np.zeros(n)
+    rc = load().fmm_make_bodies(n, dist.encode(), seed, _ptr(x), _ptr(q))
+    if rc != 0:
+        raise FMMError("fmm_make_bodies: unknown distribution %r (rc=%d)" % (dist, rc))
+    return x, q
 
 
 def _ptr(a, t=_dp):
@@ -184,6 +198,13 @@ class FMM:
         trg = np.zeros((self.n, 4))
         self._check(load().fmm_get_targets(self._h, _ptr(trg), 1 if input_order else 0),
                     "fmm_get_targets")
+        return trg
+
+    def direct(self, xt):
+        """Traversal::direct at the points xt against all bodies (GPU)."""
+        xt = np.ascontiguousarray(xt, dtype=np.float64).reshape(-1, 3)
+        trg = np.zeros((len(xt), 4))
+        self._check(load().fmm_direct(self._h, len(xt), _ptr(xt), _ptr(trg)), "fmm_direct")
         return trg
 
     def stats(self):

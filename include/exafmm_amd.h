@@ -55,10 +55,19 @@ typedef struct {
   double host_bounds_ms, host_tree_ms, host_lists_ms, host_upload_ms;
   /* device times of the last evaluation (ms, HIP events on the ctx stream) */
   double dev_upward_ms, dev_p2p_ms, dev_m2l_ms, dev_downward_ms, dev_total_ms;
+  /* the M2L and P2P kernels alone (no memsets), same events */
+  double dev_m2l_kernel_ms, dev_p2p_kernel_ms;
 } fmm_stats;
 
 /* Fill p with the reference defaults (include/args.h:188-205). */
 void fmm_default_params(fmm_params *p);
+
+/* Dataset::initBodies (include/dataset.h:230-255): seeded drand48 inputs in
+ * the reference's draw order.  dist: "cube", "sphere", "octant", "plummer",
+ * "lattice" (the reference's generators) or "cube1" ([-1,1]^3, zero-mean
+ * q in [-1,1]) and "plummer1" (unit-scale Plummer truncated at r=10, q=1/n).
+ * No device needed. */
+int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q);
 
 /* ---- host-only plan (no device needed) ----
  * BoundBox::getBounds + BuildTree::buildTree + the pair lists of
@@ -105,6 +114,12 @@ int fmm_downward(fmm_ctx *c);
 /* upward + p2p + m2l + downward, with per-phase device timing */
 int fmm_evaluate(fmm_ctx *c);
 int fmm_synchronize(fmm_ctx *c);
+
+/* Traversal::direct with images = 0 (include/traversal.h:355-377): direct
+ * sum over all bodies of the context (tree order) at nt target points xt
+ * (nt x 3); trg (nt x 4) is overwritten.  R^2 = 0 pairs are skipped like in
+ * Kernel::P2P.  Uses IEEE sqrt and division, independently of the P2P kernel. */
+int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg);
 
 /* Targets (n x 4). input_order != 0 returns them in the order of the x passed
  * to fmm_set_bodies, else in tree order (exafmm-beta leaves bodies sorted). */

@@ -108,6 +108,8 @@ def ref():
         R.ref_eval.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
                                _dp, _dp, ctypes.c_long, _ip, _ip, _ip, _dp, _dp, _dp]
         R.ref_init_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        R.ref_time.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
+                               _dp, _dp, ctypes.c_long, _ip, _ip, ctypes.c_long, _ip, _ip, _dp, _dp]
         _ref = R
     return _ref
 
@@ -282,6 +284,23 @@ class Tree:
                    _ptr(cj, _ip), _ptr(ty, _ip), _ptr(trg), _ptr(M.view(np.float64)),
                    _ptr(Lc.view(np.float64)))
         return (trg, M, Lc) if want_ml else trg
+
+
+def ref_time(tree, ref_kernel, x_sorted, q_sorted, m2l_pairs, p2p_pairs):
+    """Time the reference operators (upward, given M2L pairs, given P2P pairs,
+    downward) on one core; returns seconds (upward, m2l, p2p, downward)."""
+    R = ref()
+    t = tree
+    xs, qs = _c(x_sorted), _c(q_sorted)
+    trg = np.zeros((t.n, 4))
+    mi, mj = (_c(a, np.int32) for a in m2l_pairs)
+    pi, pj = (_c(a, np.int32) for a in p2p_pairs)
+    times = np.zeros(4)
+    R.ref_time(ref_kernel.h, t.n, _ptr(xs), _ptr(qs), t.ncell, _ptr(t.iparent, _ip),
+               _ptr(t.ichild, _ip), _ptr(t.nchild, _ip), _ptr(t.ibody, _ip), _ptr(t.nbody, _ip),
+               _ptr(_c(t.X)), _ptr(t.R), len(mi), _ptr(mi, _ip), _ptr(mj, _ip), len(pi),
+               _ptr(pi, _ip), _ptr(pj, _ip), _ptr(trg), _ptr(times))
+    return tuple(times)
 
 
 def direct(kernel, xi, xj, qj):

@@ -11,6 +11,7 @@
 #include "types.h"
 #include "kernel.h"
 #include "dataset.h"
+#include <chrono>
 #include <cstring>
 
 using namespace exafmm_laplace;
@@ -196,6 +197,52 @@ void ref_eval(void *kp, int n, const double *x, const double *q, int ncell, cons
     for (int c = 0; c < ncell; c++) add_out(cells[c].M, Mout + 2 * (size_t)c * K.NTERM, K.NTERM);
   if (Lout)
     for (int c = 0; c < ncell; c++) add_out(cells[c].L, Lout + 2 * (size_t)c * K.NTERM, K.NTERM);
+}
+
+// Timed variant for the CPU baseline: upward pass, the M2L pairs, the P2P
+// pairs and the downward pass, each timed separately (wall clock, one core).
+// times[0..3] = upward, M2L, P2P, downward seconds.
+void ref_time(void *kp, int n, const double *x, const double *q, int ncell, const int *iparent,
+              const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+              const double *X, const double *R, long nm2l, const int *mi, const int *mj, long np2p,
+              const int *pi, const int *pj, double *trg, double *times) {
+  using clk = std::chrono::steady_clock;
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies bodies;
+  load_bodies(bodies, n, x, q);
+  Cells cells(ncell);
+  for (int c = 0; c < ncell; c++) {
+    set_cell(cells[c], X + 3 * c, R[c], K.NTERM);
+    cells[c].IPARENT = iparent[c];
+    cells[c].ICHILD = ichild[c];
+    cells[c].NCHILD = nchild[c];
+    cells[c].IBODY = ibody[c];
+    cells[c].NBODY = nbody[c];
+    cells[c].BODY = bodies.begin() + ibody[c];
+  }
+  if (ncell == 0) return;
+  Pass pass{K, cells};
+  K.Xperiodic = 0;
+  auto t0 = clk::now();
+  pass.post(cells.begin());
+  auto t1 = clk::now();
+  for (long p = 0; p < nm2l; p++) K.M2L(cells.begin() + mi[p], cells.begin() + mj[p]);
+  auto t2 = clk::now();
+  for (long p = 0; p < np2p; p++) K.P2P(cells.begin() + pi[p], cells.begin() + pj[p]);
+  auto t3 = clk::now();
+  C_iter C0 = cells.begin();
+  if (C0->NCHILD == 0) K.L2P(C0);
+  for (C_iter CC = C0 + C0->ICHILD; CC != C0 + C0->ICHILD + C0->NCHILD; CC++) pass.pre(CC);
+  auto t4 = clk::now();
+  for (int i = 0; i < n; i++)
+    for (int c = 0; c < 4; c++) trg[4 * i + c] += bodies[i].TRG[c];
+  auto sec = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+  };
+  times[0] = sec(t0, t1);
+  times[1] = sec(t1, t2);
+  times[2] = sec(t2, t3);
+  times[3] = sec(t3, t4);
 }
 
 // Dataset::initBodies (include/dataset.h:230-255): positions + initSource.
