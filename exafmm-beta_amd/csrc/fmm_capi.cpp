@@ -55,6 +55,11 @@ struct fmm_ctx {
   DBuf M, L;
   DBuf leaves, internal, level_cells, body_leaf;
   DBuf m2l_ptr, m2l_src, m2l_tgt, p2p_ptr, p2p_src, p2p_tgt;
+  // offset-grouped M2L
+  bool use_mfma = false;
+  int nchunk = 0;
+  fmm::M2LGroups groups;
+  DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
 };
 
 struct fmm_plan {
@@ -148,6 +153,7 @@ void fmm_default_params(fmm_params *p) {
   p->eps2 = 0.0;
   p->precision = FMM_FP64;
   p->device = 0;
+  p->m2l_mode = FMM_M2L_AUTO;
 }
 
 int fmm_create(fmm_ctx **out, const fmm_params *p) {
@@ -166,6 +172,9 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
   } else if (p->images != 0) {
     c->err = "periodic images are not supported (images must be 0)";
     rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET) {
+    c->err = "invalid m2l_mode";
+    rc = FMM_ERR_ARG;
   } else if (p->precision != FMM_FP64) {
     c->err = "unsupported precision mode";
     rc = FMM_ERR_UNSUPPORTED;
@@ -209,7 +218,8 @@ void fmm_destroy(fmm_ctx *c) {
                   &c->ay,     &c->az,     &c->cx,          &c->cy,        &c->cz,      &c->ibody,
                   &c->nbody,  &c->ichild, &c->nchild,      &c->iparent,   &c->M,       &c->L,
                   &c->leaves, &c->internal, &c->level_cells, &c->body_leaf, &c->m2l_ptr, &c->m2l_src,
-                  &c->m2l_tgt, &c->p2p_ptr, &c->p2p_src,     &c->p2p_tgt};
+                  &c->m2l_tgt, &c->p2p_ptr, &c->p2p_src,     &c->p2p_tgt,   &c->g_tgt,
+                  &c->g_src,  &c->g_dx,   &c->g_chunk,     &c->r_ptr,     &c->r_src,   &c->r_tgt};
   for (DBuf *b : bufs) release(*b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -403,6 +413,16 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
   UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
+  c->use_mfma = c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_mfma_supported(c->p.P) && L.nm2l > 0;
+  c->groups = fmm::M2LGroups();
+  c->nchunk = 0;
+  if (c->use_mfma) {
+    c->groups = fmm::build_m2l_groups(t, L, 256);
+    const fmm::M2LGroups &G = c->groups;
+    c->nchunk = (int)(G.chunk.size() / 4);
+    UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_dx, G.dx) UP(g_chunk, G.chunk)
+    UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
+  }
 #undef UP
   HIPCHK(hipStreamSynchronize(c->st));  // host staging vectors die here
   if ((rc = ensure(c, c->pot, sizeof(double) * n))) return rc;
@@ -423,6 +443,9 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   s.p2p_pairs = L.np2p;
   s.p2p_body_pairs = L.p2p_body_pairs;
   s.host_upload_ms = t4 - t3;
+  s.m2l_groups = c->groups.ngroup;
+  s.m2l_ungrouped = c->use_mfma ? c->groups.nrest : L.nm2l;
+  s.m2l_path = c->use_mfma ? FMM_M2L_AUTO : FMM_M2L_TARGET;
   c->have_bodies = true;
   return FMM_OK;
 }
@@ -481,10 +504,21 @@ int fmm_m2l(fmm_ctx *c) {
   HIPCHK(hipMemsetAsync(c->L.p, 0, sizeof(double2) * nterm(c) * c->tree.ncell, c->st));
   const double3 xper = make_double3(0.0, 0.0, 0.0);
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
-  HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->m2l_tgt.as<int>(),
-                            (int)c->lists.m2l_targets.size(), c->m2l_ptr.as<int>(),
-                            c->m2l_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(),
-                            c->st));
+  if (c->use_mfma) {
+    HIPCHK(fmmdev::launch_m2l_mfma(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
+                                   c->g_tgt.as<int>(), c->g_src.as<int>(), c->M.as<double2>(),
+                                   c->L.as<double2>(), c->st));
+    if (c->groups.nrest > 0)
+      HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->r_tgt.as<int>(),
+                                (int)c->groups.rest_targets.size(), c->r_ptr.as<int>(),
+                                c->r_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(), 1,
+                                c->st));
+  } else {
+    HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->m2l_tgt.as<int>(),
+                              (int)c->lists.m2l_targets.size(), c->m2l_ptr.as<int>(),
+                              c->m2l_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(), 0,
+                              c->st));
+  }
   if (c->timing) HIPCHK(hipEventRecord(c->kev[1], c->st));
   return FMM_OK;
 }

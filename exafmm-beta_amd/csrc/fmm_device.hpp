@@ -33,9 +33,17 @@ hipError_t launch_p2m(int P, DevBodies B, DevCells C, const int *leaves, int nle
                       hipStream_t s);
 // M2M into every parent listed in `cells` from its children (laplace.h:254-290)
 hipError_t launch_m2m(int P, DevCells C, const int *cells, int ncell, double2 *M, hipStream_t s);
-// M2L: L[t] = sum over the CSR list of t (laplace.h:292-322); overwrites L[t]
+// M2L: L[t] (=|+=) sum over the CSR list of t (laplace.h:292-322), one
+// wavefront per target, harmonics generated per pair (deterministic order)
 hipError_t launch_m2l(int P, DevCells C, const int *targets, int ntarget, const int *ptr,
-                      const int *src, double3 xper, const double2 *M, double2 *L, hipStream_t s);
+                      const int *src, double3 xper, const double2 *M, double2 *L, int accumulate,
+                      hipStream_t s);
+// M2L over offset groups on FP64 MFMA: chunks are (group, begin, count, 0) over
+// the grouped pair arrays; adds into L with atomics (L must hold its initial
+// value).  Available for the orders where m2l_mfma_supported(P).
+bool m2l_mfma_supported(int P);
+hipError_t launch_m2l_mfma(int P, const int4 *chunks, int nchunk, const double *gdx, const int *ptgt,
+                           const int *psrc, const double2 *M, double2 *L, hipStream_t s);
 // L2L into every cell in `cells` from its parent (laplace.h:324-357); accumulates
 hipError_t launch_l2l(int P, DevCells C, const int *cells, int ncell, double2 *L, hipStream_t s);
 // L2P for every body (laplace.h:359-388); accumulates into the targets

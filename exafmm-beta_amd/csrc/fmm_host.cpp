@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 namespace fmm {
 
@@ -281,6 +282,154 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3])
     bp += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
   L.p2p_body_pairs = bp;
   return L;
+}
+
+namespace {
+
+// Open-addressing hash of 64-bit keys to dense ids (first-appearance order).
+struct KeyIndex {
+  std::vector<uint64_t> keys;
+  std::vector<int> ids;
+  size_t mask = 0;
+  int count = 0;
+  static constexpr uint64_t kEmpty = ~0ull;
+  explicit KeyIndex(size_t cap = 1 << 12) { rehash(cap); }
+  static inline uint64_t mix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+  }
+  void rehash(size_t cap) {
+    std::vector<uint64_t> ok;
+    std::vector<int> oi;
+    ok.swap(keys);
+    oi.swap(ids);
+    keys.assign(cap, kEmpty);
+    ids.assign(cap, -1);
+    mask = cap - 1;
+    for (size_t i = 0; i < ok.size(); i++)
+      if (ok[i] != kEmpty) place(ok[i], oi[i]);
+  }
+  void place(uint64_t k, int id) {
+    size_t h = mix(k) & mask;
+    while (keys[h] != kEmpty) h = (h + 1) & mask;
+    keys[h] = k;
+    ids[h] = id;
+  }
+  int get(uint64_t k, bool &fresh) {
+    size_t h = mix(k) & mask;
+    while (keys[h] != kEmpty) {
+      if (keys[h] == k) {
+        fresh = false;
+        return ids[h];
+      }
+      h = (h + 1) & mask;
+    }
+    fresh = true;
+    keys[h] = k;
+    ids[h] = count++;
+    if (2 * (size_t)count > keys.size()) rehash(2 * keys.size());
+    return count - 1;
+  }
+};
+
+}  // namespace
+
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
+  M2LGroups G;
+  const int nc = t.ncell;
+  G.rest_ptr.assign(nc + 1, 0);
+  if (nc == 0 || L.nm2l == 0) {
+    G.grp_ptr.assign(1, 0);
+    return G;
+  }
+  const double R0 = t.R[0];
+  const int64_t np = L.nm2l;
+  std::vector<int> gid(np);
+  std::vector<int64_t> kvec;  // 3 per group
+  std::vector<int> gexp;      // e per group
+  KeyIndex index;
+  const int64_t lim = 1 << 18;
+  for (int c = 0; c < nc; c++) {
+    for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+      const int sc = L.m2l_src[p];
+      int e = std::max(t.level[c], t.level[sc]);
+      const double scale = std::ldexp(1.0, e) / R0;
+      int64_t k[3];
+      bool ok = e < 64;
+      for (int d = 0; d < 3 && ok; d++) {
+        const double v = (t.X[3 * (size_t)c + d] - t.X[3 * (size_t)sc + d]) * scale;
+        const double r = std::nearbyint(v);
+        if (!(std::fabs(v - r) < 1e-6) || std::fabs(r) >= (double)(lim << 8)) ok = false;
+        k[d] = (int64_t)r;
+      }
+      if (ok) {
+        while (e > 0 && ((k[0] | k[1] | k[2]) & 1) == 0) {
+          k[0] /= 2;
+          k[1] /= 2;
+          k[2] /= 2;
+          e--;
+        }
+        for (int d = 0; d < 3; d++)
+          if (k[d] <= -lim || k[d] >= lim) ok = false;
+      }
+      if (!ok) {
+        gid[p] = -1;
+        continue;
+      }
+      const uint64_t key = ((uint64_t)e << 57) | ((uint64_t)(k[0] + lim) << 38) |
+                           ((uint64_t)(k[1] + lim) << 19) | (uint64_t)(k[2] + lim);
+      bool fresh;
+      const int g = index.get(key, fresh);
+      if (fresh) {
+        kvec.insert(kvec.end(), {k[0], k[1], k[2]});
+        gexp.push_back(e);
+      }
+      gid[p] = g;
+    }
+  }
+  const int ng = index.count;
+  G.ngroup = ng;
+  G.dx.resize(3 * (size_t)ng);
+  for (int g = 0; g < ng; g++) {
+    const double u = R0 / std::ldexp(1.0, gexp[g]);
+    for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
+  }
+  // counting sort by group, CSR order kept inside a group
+  G.grp_ptr.assign(ng + 1, 0);
+  for (int64_t p = 0; p < np; p++)
+    if (gid[p] >= 0) G.grp_ptr[gid[p] + 1]++;
+  for (int g = 0; g < ng; g++) G.grp_ptr[g + 1] += G.grp_ptr[g];
+  const int64_t ngp = G.grp_ptr[ng];
+  G.tgt.resize(ngp);
+  G.src.resize(ngp);
+  std::vector<int> fill(G.grp_ptr.begin(), G.grp_ptr.end() - 1);
+  std::vector<int> rest_t, rest_s;
+  for (int c = 0; c < nc; c++) {
+    for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+      const int g = gid[p];
+      if (g >= 0) {
+        const int q = fill[g]++;
+        G.tgt[q] = c;
+        G.src[q] = L.m2l_src[p];
+      } else {
+        G.rest_ptr[c + 1]++;
+        rest_s.push_back(L.m2l_src[p]);
+      }
+    }
+  }
+  for (int c = 0; c < nc; c++) G.rest_ptr[c + 1] += G.rest_ptr[c];
+  G.rest_src = std::move(rest_s);
+  G.nrest = (int64_t)G.rest_src.size();
+  for (int c = 0; c < nc; c++)
+    if (G.rest_ptr[c + 1] > G.rest_ptr[c]) G.rest_targets.push_back(c);
+  for (int g = 0; g < ng; g++)
+    for (int b = G.grp_ptr[g]; b < G.grp_ptr[g + 1]; b += chunk_pairs)
+      G.chunk.insert(G.chunk.end(), {g, b, std::min(chunk_pairs, G.grp_ptr[g + 1] - b), 0});
+  return G;
 }
 
 namespace {

@@ -47,6 +47,24 @@ Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const d
 // 180-230, 326-352), recording pairs instead of evaluating them.
 Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]);
 
+// M2L pairs grouped by their exact offset X_t - X_s.  Offsets between cells of
+// the reference's octree are integer multiples of R_root / 2^e; a group key is
+// the canonical (e, k) with k = offset * 2^e / R_root integral, so every pair
+// of a group has the same translation operator.  Pairs are stored group by
+// group (each group's pairs in CSR order); `chunk` splits groups into work
+// items of at most `chunk_pairs` pairs (4 ints each: group, begin, count, 0).
+// Pairs whose key does not fit the packing go to the per-target lists rest_*.
+struct M2LGroups {
+  int ngroup = 0;
+  std::vector<int> tgt, src;       // grouped pairs
+  std::vector<int> grp_ptr;        // ngroup+1
+  std::vector<double> dx;          // 3 per group: representative offset
+  std::vector<int> chunk;          // 4 per chunk
+  std::vector<int> rest_ptr, rest_src, rest_targets;
+  int64_t nrest = 0;
+};
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs);
+
 // Dataset generators (include/dataset.h:29-171, 220-227): drand48 streams
 // seeded with srand48(seed), positions first, then charges after reseeding,
 // exactly in the reference's draw order.  Returns false for an unknown name.

@@ -29,6 +29,8 @@ SYMBOLS = [
 ]
 
 FMM_FP64 = 0
+M2L_AUTO = 0      # offset-grouped FP64 MFMA where supported (P <= 10)
+M2L_TARGET = 1    # per-target kernel, fixed summation order
 
 
 class FMMError(RuntimeError):
@@ -38,7 +40,7 @@ class FMMError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("P", ctypes.c_int), ("ncrit", ctypes.c_int), ("nspawn", ctypes.c_int),
                 ("images", ctypes.c_int), ("theta", ctypes.c_double), ("eps2", ctypes.c_double),
-                ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("device", ctypes.c_int), ("m2l_mode", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -51,7 +53,8 @@ class Stats(ctypes.Structure):
                 ("dev_upward_ms", ctypes.c_double), ("dev_p2p_ms", ctypes.c_double),
                 ("dev_m2l_ms", ctypes.c_double), ("dev_downward_ms", ctypes.c_double),
                 ("dev_total_ms", ctypes.c_double), ("dev_m2l_kernel_ms", ctypes.c_double),
-                ("dev_p2p_kernel_ms", ctypes.c_double)]
+                ("dev_p2p_kernel_ms", ctypes.c_double), ("m2l_groups", ctypes.c_int64),
+                ("m2l_ungrouped", ctypes.c_int64), ("m2l_path", ctypes.c_int)]
 
     def as_dict(self):
         d = {}
@@ -119,12 +122,12 @@ class FMM:
     """One evaluator context (HIP stream + device arena)."""
 
     def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=5000, eps2=0.0, device=0,
-                 precision=FMM_FP64):
+                 precision=FMM_FP64, m2l_mode=M2L_AUTO):
         L = load()
         p = Params()
         L.fmm_default_params(ctypes.byref(p))
         p.P, p.theta, p.ncrit, p.nspawn, p.eps2 = P, theta, ncrit, nspawn, eps2
-        p.device, p.precision = device, precision
+        p.device, p.precision, p.m2l_mode = device, precision, m2l_mode
         self.P = P
         self.nterm = P * (P + 1) // 2
         self._h = ctypes.c_void_p()

@@ -33,6 +33,11 @@ typedef enum {
 
 typedef enum { FMM_FP64 = 0 } fmm_precision;
 
+/* M2L execution: AUTO uses the offset-grouped FP64 MFMA kernel where the order
+ * allows (P <= 10) and the per-target kernel otherwise; TARGET always uses the
+ * per-target kernel, whose summation order is fixed (bitwise repeatable). */
+typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1 } fmm_m2l_mode;
+
 /* The knobs of exafmm-beta's Kernel / Traversal / BuildTree constructors and
  * Args defaults (include/args.h:188-205): P=10, theta=0.4, ncrit=64,
  * nspawn=5000, images=0, eps2=0. */
@@ -45,6 +50,7 @@ typedef struct {
   double eps2;    /* softening added to r^2 in P2P */
   int precision;  /* fmm_precision */
   int device;     /* HIP device ordinal */
+  int m2l_mode;   /* fmm_m2l_mode */
 } fmm_params;
 
 typedef struct {
@@ -57,6 +63,10 @@ typedef struct {
   double dev_upward_ms, dev_p2p_ms, dev_m2l_ms, dev_downward_ms, dev_total_ms;
   /* the M2L and P2P kernels alone (no memsets), same events */
   double dev_m2l_kernel_ms, dev_p2p_kernel_ms;
+  /* M2L grouping: offset groups, pairs left to the per-target kernel, path
+   * taken by the last upload (fmm_m2l_mode value actually used) */
+  int64_t m2l_groups, m2l_ungrouped;
+  int m2l_path;
 } fmm_stats;
 
 /* Fill p with the reference defaults (include/args.h:188-205). */

@@ -180,14 +180,45 @@ def test_coincident_bodies_masked():
 
 def test_input_order_and_determinism():
     x, q = _inputs(5000, "plummer", seed=2)
-    f = _solve(x, q, 8)
+    F = _fmm()
+    # per-target M2L: fixed summation order, bitwise repeatable
+    f = F.FMM(P=8, m2l_mode=F.M2L_TARGET)
+    f.set_bodies(x, q)
+    f.evaluate()
     a = f.targets(input_order=False)
     b = f.targets(input_order=True)
     perm = f.tree()["perm"]
     assert np.array_equal(b[perm], a)
     f.evaluate()
-    assert np.array_equal(f.targets(input_order=False), a)  # bitwise repeatable
+    assert np.array_equal(f.targets(input_order=False), a)
     f.close()
+    # grouped MFMA M2L accumulates with atomics: repeatable to rounding
+    g = F.FMM(P=8, m2l_mode=F.M2L_AUTO)
+    g.set_bodies(x, q)
+    g.evaluate()
+    c = g.targets(input_order=False)
+    g.evaluate()
+    assert _rel(g.targets(input_order=False), c) < 1e-14
+    assert _rel(c, a) < 1e-13
+    assert g.stats()["m2l_path"] == F.M2L_AUTO
+    g.close()
+
+
+@pytest.mark.parametrize("P", [6, 10])
+def test_grouped_m2l_matches_per_target(P):
+    x, q = _inputs(20000, "plummer", seed=4)
+    F = _fmm()
+    out = []
+    for mode in (F.M2L_TARGET, F.M2L_AUTO):
+        f = F.FMM(P=P, m2l_mode=mode)
+        f.set_bodies(x, q)
+        f.upward()
+        f.m2l()
+        out.append(f.expansions()[1])
+        st = f.stats()
+        f.close()
+    assert st["m2l_groups"] > 0 and st["m2l_ungrouped"] == 0
+    assert _rel(out[1], out[0]) < PHASE_TOL
 
 
 def test_set_charges_linearity():
