@@ -417,7 +417,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   c->groups = fmm::M2LGroups();
   c->nchunk = 0;
   if (c->use_mfma) {
-    c->groups = fmm::build_m2l_groups(t, L, 256);
+    c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs);
     const fmm::M2LGroups &G = c->groups;
     c->nchunk = (int)(G.chunk.size() / 4);
     UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_dx, G.dx) UP(g_chunk, G.chunk)

@@ -42,6 +42,7 @@ hipError_t launch_m2l(int P, DevCells C, const int *targets, int ntarget, const 
 // the grouped pair arrays; adds into L with atomics (L must hold its initial
 // value).  Available for the orders where m2l_mfma_supported(P).
 bool m2l_mfma_supported(int P);
+constexpr int kM2LChunkPairs = 512;  // pairs per M2L chunk (one block)
 hipError_t launch_m2l_mfma(int P, const int4 *chunks, int nchunk, const double *gdx, const int *ptgt,
                            const int *psrc, const double2 *M, double2 *L, hipStream_t s);
 // L2L into every cell in `cells` from its parent (laplace.h:324-357); accumulates

@@ -38,6 +38,8 @@ struct Node {
   double X[3];
 };
 
+constexpr int kMaxLevel = 30;
+
 struct Builder {
   const double *x;
   int ncrit;
@@ -65,7 +67,10 @@ struct Builder {
   int build(int *bodies, int *buffer, int begin, int end, const double *X, double R0, int level,
             bool direction) {
     if (begin == end) return -1;
-    if (end - begin <= ncrit) {
+    // The reference recurses until a node holds <= ncrit bodies; more than
+    // ncrit coincident bodies never separate and its radius arithmetic
+    // (1 << level) breaks past level 30, so stop there with an oversized leaf.
+    if (end - begin <= ncrit || level >= kMaxLevel) {
       if (direction) std::memcpy(buffer + begin, bodies + begin, sizeof(int) * (end - begin));
       return make(begin, end, X);
     }
@@ -497,18 +502,6 @@ bool make_bodies(int n, const char *dist, int seed, double *x, double *q) {
       for (int k = 0; k < n; k++) q[k] = 1.0 / n;
     else
       reference_charges(n, seed, q);
-  } else if (d == "lattice") {
-    int nx = int(std::pow(n, 1. / 3));
-    int k = 0;
-    for (int ix = 0; ix < nx; ix++)
-      for (int iy = 0; iy < nx; iy++)
-        for (int iz = 0; iz < nx; iz++, k++) {
-          x[3 * (size_t)k] = (ix / double(nx - 1)) * 2 - 1;
-          x[3 * (size_t)k + 1] = (iy / double(nx - 1)) * 2 - 1;
-          x[3 * (size_t)k + 2] = (iz / double(nx - 1)) * 2 - 1;
-        }
-    for (; k < n; k++) x[3 * (size_t)k] = x[3 * (size_t)k + 1] = x[3 * (size_t)k + 2] = 0;
-    reference_charges(n, seed, q);
   } else if (d == "cube1") {
     srand48(seed);
     for (int i = 0; i < n; i++)

@@ -65,14 +65,13 @@ struct M2LGroups {
 };
 M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs);
 
-// Dataset generators (include/dataset.h:29-171, 220-227): drand48 streams
+// Dataset generators (include/dataset.h:52-171, 220-227): drand48 streams
 // seeded with srand48(seed), positions first, then charges after reseeding,
 // exactly in the reference's draw order.  Returns false for an unknown name.
 //   "cube"     [-pi,pi]^3, q = drand48()-.5 minus mean        (dataset.h:52-66)
 //   "sphere"   |x| = pi shell                                  (dataset.h:69-87)
 //   "octant"   one octant of the pi sphere                     (dataset.h:90-106)
 //   "plummer"  Plummer, scaled by 3pi/16, R < 100              (dataset.h:109-136)
-//   "lattice"  regular [-1,1]^3 lattice (n rounded down to k^3) (dataset.h:29-49)
 //   "cube1"    [-1,1]^3, q = 2 drand48()-1 minus mean
 //   "plummer1" Plummer with unit scale radius, r < 10, q = 1/n
 bool make_bodies(int n, const char *dist, int seed, double *x, double *q);

@@ -73,8 +73,8 @@ typedef struct {
 void fmm_default_params(fmm_params *p);
 
 /* Dataset::initBodies (include/dataset.h:230-255): seeded drand48 inputs in
- * the reference's draw order.  dist: "cube", "sphere", "octant", "plummer",
- * "lattice" (the reference's generators) or "cube1" ([-1,1]^3, zero-mean
+ * the reference's draw order.  dist: "cube", "sphere", "octant", "plummer"
+ * (the reference's generators) or "cube1" ([-1,1]^3, zero-mean
  * q in [-1,1]) and "plummer1" (unit-scale Plummer truncated at r=10, q=1/n).
  * No device needed. */
 int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q);

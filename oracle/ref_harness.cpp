@@ -249,6 +249,7 @@ void ref_time(void *kp, int n, const double *x, const double *q, int ncell, cons
 void ref_init_bodies(int n, const char *dist, int seed, double *x, double *q) {
   Dataset data;
   Bodies b = data.initBodies(n, dist, seed);
+  if ((int)b.size() < n) n = (int)b.size();  // lattice returns floor(cbrt(n))^3 bodies
   for (int i = 0; i < n; i++) {
     for (int d = 0; d < 3; d++) x[3 * i + d] = b[i].X[d];
     q[i] = b[i].SRC;

@@ -1,0 +1,110 @@
+"""CPU checks of the product's host side (through the C ABI of
+libexafmm_amd.so): exported symbols, and the tree / interaction lists built by
+fmm_plan_build against the oracle (identical trees, identical pair lists in
+the reference's visiting order)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+def test_library_exports_every_declared_symbol():
+    F = _F()
+    hdr = open(os.path.join(ROOT, "include", "exafmm_amd.h")).read()
+    names = set(re.findall(r"\b(fmm_[a-z_0-9]+)\s*\(", hdr))
+    assert names, "no declarations found"
+    lib = ctypes.CDLL(F.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names == set(F.SYMBOLS) | (names - set(F.SYMBOLS))
+    assert set(F.SYMBOLS) <= names
+
+
+def _oracle_csr(t, ci, cj, ty, typ):
+    sel = ty == typ
+    a, b = ci[sel], cj[sel]
+    order = np.argsort(a, kind="stable")
+    ptr = np.zeros(t.ncell + 1, np.int64)
+    ptr[1:] = np.cumsum(np.bincount(a, minlength=t.ncell))
+    return ptr, b[order]
+
+
+@pytest.mark.parametrize("dist,n,ncrit,theta,nspawn", [
+    ("cube", 32768, 64, 0.4, 5000), ("plummer", 20000, 64, 0.4, 5000),
+    ("plummer", 20000, 8, 0.25, 100), ("cube", 5000, 16, 0.6, 1000), ("cube", 1, 64, 0.4, 5000),
+    ("cube", 64, 64, 0.4, 5000)])
+def test_plan_matches_oracle(dist, n, ncrit, theta, nspawn):
+    F = _F()
+    x = O.data_cube(n, 2) if dist == "cube" else O.data_plummer(n, 2)
+    pl = F.Plan(x, ncrit=ncrit, theta=theta, nspawn=nspawn)
+    t = O.Tree(x, ncrit)
+    tr = pl.tree()
+    assert np.array_equal(tr["perm"], t.perm)
+    for k in ("iparent", "ichild", "nchild", "ibody", "nbody"):
+        assert np.array_equal(tr[k], getattr(t, k)), k
+    assert np.array_equal(tr["X"], t.X) and np.array_equal(tr["R"], t.R)
+    assert tr["depth"] == t.depth
+    ci, cj, ty = t.traverse(theta, nspawn)
+    mp, ms, pp, ps = pl.lists()
+    optr, osrc = _oracle_csr(t, ci, cj, ty, 0)
+    assert np.array_equal(mp, optr) and np.array_equal(ms, osrc)
+    optr, osrc = _oracle_csr(t, ci, cj, ty, 1)
+    assert np.array_equal(pp, optr) and np.array_equal(ps, osrc)
+    s = pl.stats()
+    assert s["m2l_pairs"] == int((ty == 0).sum()) and s["p2p_pairs"] == int((ty == 1).sum())
+
+
+def test_plan_empty_and_duplicates():
+    F = _F()
+    pl = F.Plan(np.zeros((0, 3)))
+    assert pl.stats()["ncell"] == 0
+    x = O.data_cube(500, 3)
+    x[100:140] = x[0]  # 41 coincident bodies (<= ncrit): same tree as the reference
+    pl = F.Plan(x, ncrit=64)
+    t = O.Tree(x, 64)
+    assert pl.stats()["ncell"] == t.ncell
+    assert np.array_equal(pl.tree()["perm"], t.perm)
+    # more than ncrit coincident bodies never separate: the reference recursion
+    # does not terminate there; the product stops at level 30 with one leaf
+    x[100:300] = x[0]
+    tr = F.Plan(x, ncrit=64).tree()
+    assert tr["depth"] == 30
+    assert tr["nbody"][tr["nchild"] == 0].max() == 201
+
+
+def test_make_bodies_matches_generators():
+    F = _F()
+    x, q = F.make_bodies(4000, "cube", 3)
+    assert np.array_equal(x, O.data_cube(4000, 3)) and np.array_equal(q, O.data_source(4000, 3))
+    x, q = F.make_bodies(4000, "plummer", 3)
+    assert np.array_equal(x, O.data_plummer(4000, 3)) and np.array_equal(q, O.data_source(4000, 3))
+    if O.ref_available():
+        for d in ("cube", "sphere"):
+            a, b = F.make_bodies(4096, d, 1)
+            c, e = O.ref_init_bodies(4096, d, 1)
+            assert np.array_equal(a, c) and np.array_equal(b, e), d
+    x, q = F.make_bodies(10000, "plummer1", 0)
+    assert np.all(np.linalg.norm(x, axis=1) < 10) and np.all(q == 1e-4)
+    x, q = F.make_bodies(10000, "cube1", 0)
+    assert np.all(np.abs(x) <= 1) and abs(q.mean()) < 1e-15 and np.abs(q).max() <= 1.001
+    with pytest.raises(F.FMMError):
+        F.make_bodies(10, "nonsense", 0)
+
+
+def test_create_reports_errors():
+    F = _F()
+    with pytest.raises(F.FMMError):
+        F.FMM(P=1)            # unsupported order, rejected before any device call
+    with pytest.raises(F.FMMError):
+        F.FMM(P=10, theta=0)  # invalid parameter

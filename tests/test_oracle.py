@@ -1,0 +1,256 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each operator is checked against exafmm-beta's own compiled operators
+(oracle/_ref, when the reference tree was available at build time), against
+golden numbers the reference printed (tree/list counts and rel-L2 errors
+recorded from the reference binary), and against what the mathematics fixes
+(closed forms, translation exactness, convergence in P, linearity).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+REF = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _cluster(rng, c, r, n):
+    return c + rng.uniform(-r, r, (n, 3))
+
+
+# ---------------------------------------------------------------- operators vs reference
+@REF
+@pytest.mark.parametrize("P", [3, 4, 10, 13])
+def test_operators_match_reference(P):
+    rng = np.random.default_rng(P)
+    ko, kr = O.Kernel(P), O.Kernel(P, use_ref=True)
+    Xc = rng.uniform(-1, 1, 3)
+    xb = _cluster(rng, Xc, 0.5, 37)
+    q = rng.uniform(-1, 1, 37)
+    Mo, Mr = ko.p2m(Xc, xb, q), kr.p2m(Xc, xb, q)
+    assert _rel(Mo, Mr) < 1e-14
+    Xp = Xc + np.array([0.5, -0.5, 0.5])
+    Mpo, Mpr = ko.m2m(Xp, Xc, Mo), kr.m2m(Xp, Xc, Mr)
+    assert _rel(Mpo, Mpr) < 1e-13
+    Xi = Xp + np.array([4.0, 1.0, -3.0])
+    Lo, Lr = ko.m2l(Xi, Xp, Mpo), kr.m2l(Xi, Xp, Mpr)
+    assert _rel(Lo, Lr) < 1e-13
+    Xl = Xi + np.array([-0.25, 0.5, 0.25])
+    Llo, Llr = ko.l2l(Xl, Xi, Lo), kr.l2l(Xl, Xi, Lr)
+    assert _rel(Llo, Llr) < 1e-13
+    xt = _cluster(rng, Xl, 0.25, 21)
+    assert _rel(ko.l2p(Xl, Llo, xt), kr.l2p(Xl, Llr, xt)) < 1e-13
+    # P2P: the reference's SIMD path seeds rsqrt in float + 2 Newton steps
+    assert _rel(ko.p2p(xt, xb, q), kr.p2p(xt, xb, q)) < 1e-12
+
+
+@REF
+def test_generators_match_reference():
+    for dist, gen in (("cube", O.data_cube),):
+        x, q = O.ref_init_bodies(3000, dist, 5)
+        assert np.array_equal(gen(3000, 5), x)
+        assert np.array_equal(O.data_source(3000, 5), q)
+
+
+# ---------------------------------------------------------------- golden numbers from the reference binary
+def test_golden_counts_cube32768():
+    """examples/fmm.cxx defaults (cube, N=32768, ncrit=64, theta=.4): the reference
+    builds 2,413 cells (2,111 leaves, depth 4) and visits 312,856 M2L and
+    142,813 P2P pairs (41,663,490 body pairs)."""
+    x = O.data_cube(32768, 0)
+    t = O.Tree(x, 64)
+    assert (t.ncell, int((t.nchild == 0).sum()), t.depth) == (2413, 2111, 4)
+    ci, cj, ty = t.traverse(0.4, 5000)
+    assert int((ty == 0).sum()) == 312856 and int((ty == 1).sum()) == 142813
+    p2p = ty == 1
+    assert int((t.nbody[ci[p2p]].astype(np.int64) * t.nbody[cj[p2p]]).sum()) == 41663490
+
+
+def test_golden_counts_plummer1m():
+    """Reference Plummer generator, N=2^20: 64,513 cells, 56,446 leaves, depth 12,
+    11,883,385 M2L and 4,288,618 P2P pairs, 1,999,873,168 body pairs."""
+    x = O.data_plummer(1 << 20, 0)
+    t = O.Tree(x, 64)
+    assert (t.ncell, int((t.nchild == 0).sum()), t.depth) == (64513, 56446, 12)
+    ci, cj, ty = t.traverse(0.4, 5000)
+    assert int((ty == 0).sum()) == 11883385 and int((ty == 1).sum()) == 4288618
+    p2p = ty == 1
+    assert int((t.nbody[ci[p2p]].astype(np.int64) * t.nbody[cj[p2p]]).sum()) == 1999873168
+
+
+def test_golden_error_cube32768_p10():
+    """The reference prints rel-L2 4.7429586e-06 (pot) / 4.8780842e-06 (acc) for
+    the examples/fmm.cxx defaults at P=10 (100 sampled tree-order targets)."""
+    n = 32768
+    x, q = O.data_cube(n, 0), O.data_source(n, 0)
+    t = O.Tree(x, 64)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(10)
+    trg = t.fmm(k, xs, qs, 0.4, 5000)
+    idx = O.sample_indices(n, 100)
+    pot, acc = O.rel_l2(O.direct(k, xs[idx], xs, qs), trg[idx])
+    assert abs(pot - 4.7429586e-06) < 1e-13
+    assert abs(acc - 4.8780842e-06) < 1e-13
+
+
+@pytest.mark.parametrize("P,pot,acc", [(6, 2.29e-4, 1.14e-4), (8, 3.14e-5, 2.37e-5)])
+def test_golden_error_vs_order(P, pot, acc):
+    """Reference convergence table (N=32768 cube, theta=.4), 3 digits."""
+    n = 32768
+    x, q = O.data_cube(n, 0), O.data_source(n, 0)
+    t = O.Tree(x, 64)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(P)
+    trg = t.fmm(k, xs, qs, 0.4, 5000)
+    idx = O.sample_indices(n, 100)
+    p, a = O.rel_l2(O.direct(k, xs[idx], xs, qs), trg[idx])
+    assert abs(p - pot) / pot < 5e-3 and abs(a - acc) / acc < 5e-3
+
+
+def _kernel_chain(P):
+    """examples/kernel.cxx: one unit source through P2M -> M2M -> M2L -> L2L -> L2P."""
+    k = O.Kernel(P)
+    src = np.array([[2.0, 2.0, 2.0]])
+    M = k.p2m([3, 1, 1], src, [1.0])
+    MJ = k.m2m([4, 0, 0], [3, 1, 1], M)
+    LI = k.m2l([-4, 0, 0], [4, 0, 0], MJ)
+    Li = k.l2l([-3, 1, 1], [-4, 0, 0], LI)
+    tgt = np.array([[-2.0, 2.0, 2.0]])
+    fmm = k.l2p([-3, 1, 1], Li, tgt)
+    d = k.p2p(tgt, src, [1.0])
+    # kernel.cxx:93-97 normalises by the FMM value (getNrm* of `bodies`, the L2P result)
+    pot = abs(fmm[0, 0] - d[0, 0]) / abs(fmm[0, 0])
+    acc = np.linalg.norm(fmm[0, 1:] - d[0, 1:]) / np.linalg.norm(fmm[0, 1:])
+    return pot, acc
+
+
+@pytest.mark.parametrize("P,pot,acc", [(4, 5.05e-2, 1.50e-1), (10, 1.48e-3, 2.19e-2),
+                                       (20, 4.65e-5, 1.06e-3)])
+def test_kernel_chain_known_answer(P, pot, acc):
+    """examples/kernel.cxx prints 'P potErr accErr'; the reference gives these values."""
+    p, a = _kernel_chain(P)
+    assert abs(p - pot) / pot < 1e-2 and abs(a - acc) / acc < 1e-2
+
+
+# ---------------------------------------------------------------- mathematics
+def test_p2p_closed_form_and_brute_force():
+    k = O.Kernel(4)
+    y = np.array([[0.3, -0.2, 0.1]])
+    x = np.array([[1.0, 2.0, -0.5]])
+    d = x[0] - y[0]
+    r = np.linalg.norm(d)
+    t = k.p2p(x, y, [2.0])[0]
+    assert abs(t[0] - 2.0 / r) < 1e-15
+    assert np.allclose(t[1:], -2.0 * d / r**3, rtol=1e-15, atol=0)
+    rng = np.random.default_rng(0)
+    xi, xj, qj = rng.normal(size=(200, 3)), rng.normal(size=(300, 3)), rng.normal(size=300)
+    dx = xi[:, None, :] - xj[None, :, :]
+    r = np.linalg.norm(dx, axis=2)
+    pot = (qj / r).sum(1)
+    grad = -(qj[None, :, None] * dx / r[:, :, None] ** 3).sum(1)
+    t = k.p2p(xi, xj, qj)
+    assert _rel(t[:, 0], pot) < 1e-14 and _rel(t[:, 1:], grad) < 1e-14
+    # R^2 = 0 pairs are skipped
+    assert np.array_equal(k.p2p(y, y, [1.0]), np.zeros((1, 4)))
+
+
+def test_p2m_closed_form():
+    """P2M of a unit charge on the z axis: M[n,0] = rho^n (P_n(1) = 1, prefactor 1)."""
+    P = 8
+    k = O.Kernel(P)
+    M = k.p2m([0, 0, 0], [[0.0, 0.0, 0.5]], [1.0])
+    for n in range(P):
+        assert abs(M[n * (n + 1) // 2] - 0.5**n) < 1e-15
+        for m in range(1, n + 1):
+            assert abs(M[n * (n + 1) // 2 + m]) < 1e-15
+
+
+def test_m2m_is_exact_translation():
+    """P2M at the parent equals M2M of the child's P2M (no truncation upward)."""
+    rng = np.random.default_rng(7)
+    k = O.Kernel(9)
+    Xc, Xp = np.array([0.25, 0.25, -0.25]), np.zeros(3)
+    xb = _cluster(rng, Xc, 0.2, 30)
+    q = rng.uniform(-1, 1, 30)
+    assert _rel(k.m2m(Xp, Xc, k.p2m(Xc, xb, q)), k.p2m(Xp, xb, q)) < 1e-13
+
+
+def test_l2l_is_exact_translation():
+    """A local expansion shifted by L2L evaluates to the same field (L2P)."""
+    rng = np.random.default_rng(8)
+    k = O.Kernel(9)
+    M = k.p2m([5, 5, 5], _cluster(rng, np.array([5.0, 5, 5]), 0.3, 20), rng.uniform(-1, 1, 20))
+    Lp = k.m2l([0, 0, 0], [5, 5, 5], M)
+    Xc = np.array([0.2, -0.1, 0.15])
+    xt = _cluster(rng, Xc, 0.1, 10)
+    assert _rel(k.l2p(Xc, k.l2l(Xc, [0, 0, 0], Lp), xt), k.l2p([0, 0, 0], Lp, xt)) < 1e-12
+
+
+def test_m2l_converges_geometrically():
+    rng = np.random.default_rng(9)
+    src = _cluster(rng, np.array([3.0, 0, 0]), 0.5, 10)
+    q = rng.uniform(-1, 1, 10)
+    tgt = _cluster(rng, np.zeros(3), 0.5, 10)
+    d = O.Kernel(4).p2p(tgt, src, q)
+    errs = []
+    for P in (4, 8, 12, 16):
+        k = O.Kernel(P)
+        L = k.m2l([0, 0, 0], [3, 0, 0], k.p2m([3, 0, 0], src, q))
+        errs.append(_rel(k.l2p([0, 0, 0], L, tgt)[:, 0], d[:, 0]))
+    assert all(b < a / 5 for a, b in zip(errs, errs[1:])), errs
+
+
+def test_fmm_single_leaf_and_small_theta_are_direct():
+    x, q = O.data_cube(50, 3), O.data_source(50, 3)
+    t = O.Tree(x, 64)
+    assert t.ncell == 1
+    k = O.Kernel(6)
+    xs, qs = x[t.perm], q[t.perm]
+    assert _rel(t.fmm(k, xs, qs), O.direct(k, xs, xs, qs)) < 1e-14
+    x, q = O.data_plummer(1500, 4), O.data_source(1500, 4)
+    t = O.Tree(x, 16)
+    ci, cj, ty = t.traverse(1e-6, 5000)
+    assert int((ty == 0).sum()) == 0
+    xs, qs = x[t.perm], q[t.perm]
+    assert _rel(t.fmm(k, xs, qs, 1e-6), O.direct(k, xs, xs, qs)) < 1e-13
+
+
+def test_fmm_linear_in_charges():
+    x = O.data_cube(3000, 1)
+    rng = np.random.default_rng(1)
+    q1, q2 = rng.uniform(-1, 1, 3000), rng.uniform(-1, 1, 3000)
+    t = O.Tree(x, 64)
+    k = O.Kernel(6)
+    xs = x[t.perm]
+    a = t.fmm(k, xs, q1[t.perm])
+    b = t.fmm(k, xs, q2[t.perm])
+    c = t.fmm(k, xs, (q1 + 2 * q2)[t.perm])
+    assert _rel(c, a + 2 * b) < 1e-13
+
+
+def test_bounds_drift():
+    """BoundBox::getBounds widens the box by 1e-5 per body (include/bound_box.h:19-20)."""
+    x = np.tile([[0.5, -0.5, 0.0]], (10, 1))
+    lo, hi = O.bounds(x)
+    assert np.allclose(lo, x[0] - 10e-5, rtol=0, atol=1e-15)
+    assert np.allclose(hi, x[0] + 10e-5, rtol=0, atol=1e-15)
+
+
+def test_tree_invariants():
+    x = O.data_plummer(20000, 2)
+    t = O.Tree(x, 32)
+    assert np.array_equal(np.sort(t.perm), np.arange(len(x)))
+    leaf = t.nchild == 0
+    assert t.nbody[leaf].max() <= 32 and t.nbody[leaf].sum() == len(x)
+    for c in np.nonzero(~leaf)[0]:
+        ch = np.arange(t.ichild[c], t.ichild[c] + t.nchild[c])
+        assert np.all(t.iparent[ch] == c)
+        assert t.nbody[ch].sum() == t.nbody[c]
+        assert np.allclose(t.R[ch], t.R[c] / 2)
