@@ -5,6 +5,7 @@
 #include "fmm_host.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -431,9 +432,27 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
   G.nrest = (int64_t)G.rest_src.size();
   for (int c = 0; c < nc; c++)
     if (G.rest_ptr[c + 1] > G.rest_ptr[c]) G.rest_targets.push_back(c);
+  // Chunks sorted by their first target (cell index ranges are subtrees, so
+  // spatially compact), then dealt to the XCDs in contiguous runs: block b of
+  // the launch runs on XCD b % kXcd, so XCD x gets the x-th eighth of the
+  // domain and its concurrent chunks share source multipoles in its L2.
+  // Empty chunks pad the shorter runs.
+  std::vector<std::array<int, 4>> ch;
   for (int g = 0; g < ng; g++)
     for (int b = G.grp_ptr[g]; b < G.grp_ptr[g + 1]; b += chunk_pairs)
-      G.chunk.insert(G.chunk.end(), {g, b, std::min(chunk_pairs, G.grp_ptr[g + 1] - b), 0});
+      ch.push_back({g, b, std::min(chunk_pairs, G.grp_ptr[g + 1] - b), 0});
+  std::stable_sort(ch.begin(), ch.end(), [&](const std::array<int, 4> &a, const std::array<int, 4> &b) {
+    return G.tgt[a[1]] < G.tgt[b[1]];
+  });
+  const int kXcd = 8;
+  const size_t nch = ch.size(), per = (nch + kXcd - 1) / kXcd;
+  G.chunk.assign(4 * per * kXcd, 0);
+  for (size_t i = 0; i < per; i++)
+    for (int x = 0; x < kXcd; x++) {
+      const size_t src = (size_t)x * per + i, dst = i * kXcd + x;
+      if (src < nch)
+        for (int k = 0; k < 4; k++) G.chunk[4 * dst + k] = ch[src][k];
+    }
   return G;
 }
 
