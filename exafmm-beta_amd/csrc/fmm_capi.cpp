@@ -451,6 +451,77 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
 }
 }  // namespace
 
+}  // extern "C"
+
+namespace {
+// caller-built tree -> plan (lists built here); returns "" or the reason it was refused
+std::string plan_from_tree(fmm_plan &pl, int n, const double *x, int ncell, const int *iparent,
+                           const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+                           const double *X, const double *R, double theta, int nspawn) {
+  if (n < 0 || ncell < 0 || (n > 0 && !x) ||
+      (ncell > 0 && (!iparent || !ichild || !nchild || !ibody || !nbody || !X || !R)))
+    return "bad sizes or NULL array";
+  if (!(theta > 0) || nspawn < 1) return "theta must be > 0 and nspawn >= 1";
+  pl.n = n;
+  const double t0 = now_ms();
+  const std::string why = fmm::adopt_tree(pl.tree, n, ncell, iparent, ichild, nchild, ibody, nbody, X, R);
+  if (!why.empty()) return why;
+  const double t1 = now_ms();
+  const double xper[3] = {0, 0, 0};
+  pl.lists = fmm::build_lists(pl.tree, theta, nspawn, xper);
+  pl.t_tree = t1 - t0;
+  pl.t_lists = now_ms() - t1;
+  fmm::get_bounds(n, x, pl.xmin, pl.xmax);
+  return "";
+}
+}  // namespace
+
+extern "C" {
+
+int fmm_plan_from_tree(fmm_plan **out, int n, const double *x, int ncell, const int *iparent,
+                       const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+                       const double *X, const double *R, double theta, int nspawn, char *why,
+                       int why_len) {
+  if (!out) return FMM_ERR_ARG;
+  *out = nullptr;
+  fmm_plan *pl = new (std::nothrow) fmm_plan();
+  if (!pl) return FMM_ERR_MEMORY;
+  std::string msg;
+  try {
+    msg = plan_from_tree(*pl, n, x, ncell, iparent, ichild, nchild, ibody, nbody, X, R, theta, nspawn);
+  } catch (const std::bad_alloc &) {
+    delete pl;
+    return FMM_ERR_MEMORY;
+  }
+  if (!msg.empty()) {
+    delete pl;
+    if (why && why_len > 0) {
+      std::strncpy(why, msg.c_str(), (size_t)why_len - 1);
+      why[why_len - 1] = 0;
+    }
+    return FMM_ERR_ARG;
+  }
+  *out = pl;
+  return FMM_OK;
+}
+
+int fmm_set_tree(fmm_ctx *c, int n, const double *x, const double *q, int ncell,
+                 const int *iparent, const int *ichild, const int *nchild, const int *ibody,
+                 const int *nbody, const double *X, const double *R) {
+  if (!c) return FMM_ERR_ARG;
+  if (n > 0 && !q) FMM_FAIL(FMM_ERR_ARG, "fmm_set_tree: NULL q");
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  fmm_plan pl;
+  try {
+    const std::string why = plan_from_tree(pl, n, x, ncell, iparent, ichild, nchild, ibody, nbody, X, R,
+                                           c->p.theta, c->p.nspawn);
+    if (!why.empty()) FMM_FAIL(FMM_ERR_ARG, "fmm_set_tree: " + why);
+  } catch (const std::bad_alloc &) {
+    FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while adopting the tree");
+  }
+  return upload_plan(c, std::move(pl), x, q);
+}
+
 int fmm_set_charges(fmm_ctx *c, const double *q) {
   int rc = check_ready(c);
   if (rc) return rc;

@@ -185,6 +185,66 @@ Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const d
   return t;
 }
 
+std::string adopt_tree(Tree &t, int n, int ncell, const int *iparent, const int *ichild,
+                       const int *nchild, const int *ibody, const int *nbody, const double *X,
+                       const double *R) {
+  t = Tree();
+  t.n = n;
+  t.ncell = ncell;
+  t.perm.resize(n);
+  for (int i = 0; i < n; i++) t.perm[i] = i;
+  if (ncell == 0) {
+    t.level_ptr.assign(1, 0);
+    return n == 0 ? "" : "bodies without cells";
+  }
+  t.iparent.assign(iparent, iparent + ncell);
+  t.ichild.assign(ichild, ichild + ncell);
+  t.nchild.assign(nchild, nchild + ncell);
+  t.ibody.assign(ibody, ibody + ncell);
+  t.nbody.assign(nbody, nbody + ncell);
+  t.X.assign(X, X + 3 * (size_t)ncell);
+  t.R.assign(R, R + ncell);
+  t.level.assign(ncell, -1);
+  if (t.ibody[0] != 0 || t.nbody[0] != n) return "root must hold all bodies";
+  t.level[0] = 0;
+  int64_t leaf_bodies = 0;
+  // children are numbered after their parent, so one pass in cell order
+  // assigns every level (cells out of order are rejected below)
+  for (int c = 0; c < ncell; c++) {
+    if (t.level[c] < 0) return "cell " + std::to_string(c) + " is not reached from the root";
+    if (t.nbody[c] < 0 || t.ibody[c] < 0 || (int64_t)t.ibody[c] + t.nbody[c] > n)
+      return "cell " + std::to_string(c) + " has a body range outside [0, n)";
+    if (t.nchild[c] < 0 || t.nchild[c] > 8) return "cell " + std::to_string(c) + " has a bad NCHILD";
+    if (t.nchild[c] == 0) {
+      leaf_bodies += t.nbody[c];
+      t.leaves.push_back(c);
+      continue;
+    }
+    if (t.ichild[c] <= c || (int64_t)t.ichild[c] + t.nchild[c] > ncell)
+      return "cell " + std::to_string(c) + " has a bad child block";
+    // the children's body ranges tile the parent's range in order
+    int64_t next = t.ibody[c];
+    for (int k = t.ichild[c]; k < t.ichild[c] + t.nchild[c]; k++) {
+      if (t.level[k] >= 0 || t.iparent[k] != c)
+        return "cell " + std::to_string(k) + " does not name " + std::to_string(c) + " as parent";
+      if (t.ibody[k] != next) return "children of cell " + std::to_string(c) + " do not tile its bodies";
+      t.level[k] = t.level[c] + 1;
+      next += t.nbody[k];
+    }
+    if (next != (int64_t)t.ibody[c] + t.nbody[c])
+      return "children of cell " + std::to_string(c) + " do not tile its bodies";
+  }
+  if (leaf_bodies != n) return "leaves do not cover every body once";
+  for (int c = 0; c < ncell; c++) t.depth = std::max(t.depth, t.level[c]);
+  t.level_ptr.assign(t.depth + 2, 0);
+  for (int c = 0; c < ncell; c++) t.level_ptr[t.level[c] + 1]++;
+  for (int l = 0; l <= t.depth; l++) t.level_ptr[l + 1] += t.level_ptr[l];
+  t.level_cells.resize(ncell);
+  std::vector<int> fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
+  for (int c = 0; c < ncell; c++) t.level_cells[fill[t.level[c]]++] = c;
+  return "";
+}
+
 namespace {
 
 struct Walker {

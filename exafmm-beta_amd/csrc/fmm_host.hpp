@@ -4,6 +4,7 @@
 // compared with the reference FMM at the same P, theta and ncrit.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace fmm {
@@ -42,6 +43,13 @@ void get_bounds(int n, const double *x, double xmin[3], double xmax[3]);
 // BuildTree::buildTree (include/build_tree.h:168-209).  x is (n,3) in input
 // order.  Returns the tree; perm gives the body order.
 Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const double xmax[3]);
+
+// Wrap a caller-built tree (reference cell order) in a Tree: checks its
+// structure and fills levels, level index and leaves.  Returns an empty
+// string on success, else the reason it was rejected.
+std::string adopt_tree(Tree &t, int n, int ncell, const int *iparent, const int *ichild,
+                       const int *nchild, const int *ibody, const int *nbody, const double *X,
+                       const double *R);
 
 // Traversal::traverse with images = 0 (include/traversal.h:117-177, 90-114,
 // 180-230, 326-352), recording pairs instead of evaluating them.

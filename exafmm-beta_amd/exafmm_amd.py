@@ -25,7 +25,7 @@ SYMBOLS = [
     "fmm_synchronize", "fmm_get_targets", "fmm_get_expansions", "fmm_get_stats", "fmm_get_tree",
     "fmm_get_lists", "fmm_set_bodies_plan", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
     "fmm_plan_tree", "fmm_plan_lists", "fmm_make_bodies",
-    "fmm_direct",
+    "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree",
 ]
 
 FMM_FP64 = 0
@@ -100,6 +100,11 @@ def load():
         L.fmm_plan_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
         L.fmm_make_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         L.fmm_direct.argtypes = [vp, ctypes.c_int, _dp, _dp]
+        L.fmm_set_tree.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
+                                   _dp, _dp]
+        L.fmm_plan_from_tree.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
+                                         _ip, _ip, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -174,6 +179,21 @@ class FMM:
             raise ValueError("x and q lengths differ")
         self.n = len(x)
         self._check(load().fmm_set_bodies(self._h, self.n, _ptr(x), _ptr(q)), "fmm_set_bodies")
+
+    def set_tree(self, x_sorted, q_sorted, tree):
+        """Caller-built tree (reference cell order); bodies already in tree order.
+        tree: dict with iparent, ichild, nchild, ibody, nbody (int), X (ncell,3), R."""
+        x = np.ascontiguousarray(x_sorted, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q_sorted, dtype=np.float64).reshape(-1)
+        a = {k: np.ascontiguousarray(tree[k], dtype=np.int32)
+             for k in ("iparent", "ichild", "nchild", "ibody", "nbody")}
+        X = np.ascontiguousarray(tree["X"], dtype=np.float64).reshape(-1, 3)
+        R = np.ascontiguousarray(tree["R"], dtype=np.float64).reshape(-1)
+        self.n = len(x)
+        self._check(load().fmm_set_tree(self._h, len(x), _ptr(x), _ptr(q), len(R),
+                                        _ptr(a["iparent"], _ip), _ptr(a["ichild"], _ip),
+                                        _ptr(a["nchild"], _ip), _ptr(a["ibody"], _ip),
+                                        _ptr(a["nbody"], _ip), _ptr(X), _ptr(R)), "fmm_set_tree")
 
     def set_charges(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
@@ -252,14 +272,28 @@ class FMM:
 class Plan:
     """Host-only tree + interaction lists (fmm_plan_*): no device needed."""
 
-    def __init__(self, x, ncrit=64, theta=0.4, nspawn=5000):
+    def __init__(self, x, ncrit=64, theta=0.4, nspawn=5000, tree=None):
+        """Build the tree of x, or, with tree= (dict of iparent, ichild, nchild,
+        ibody, nbody, X, R; x already in tree order), adopt that tree."""
         L = load()
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         self.x = x
         self._h = ctypes.c_void_p()
-        rc = L.fmm_plan_build(ctypes.byref(self._h), len(x), _ptr(x), ncrit, theta, nspawn)
+        if tree is None:
+            rc = L.fmm_plan_build(ctypes.byref(self._h), len(x), _ptr(x), ncrit, theta, nspawn)
+            if rc != 0:
+                raise FMMError("fmm_plan_build failed (rc=%d)" % rc)
+            return
+        a = {k: np.ascontiguousarray(tree[k], dtype=np.int32)
+             for k in ("iparent", "ichild", "nchild", "ibody", "nbody")}
+        X = np.ascontiguousarray(tree["X"], dtype=np.float64).reshape(-1, 3)
+        R = np.ascontiguousarray(tree["R"], dtype=np.float64).reshape(-1)
+        why = ctypes.create_string_buffer(256)
+        rc = L.fmm_plan_from_tree(ctypes.byref(self._h), len(x), _ptr(x), len(R), _ptr(a["iparent"], _ip),
+                                  _ptr(a["ichild"], _ip), _ptr(a["nchild"], _ip), _ptr(a["ibody"], _ip),
+                                  _ptr(a["nbody"], _ip), _ptr(X), _ptr(R), theta, nspawn, why, 256)
         if rc != 0:
-            raise FMMError("fmm_plan_build failed (rc=%d)" % rc)
+            raise FMMError("fmm_plan_from_tree failed (rc=%d): %s" % (rc, why.value.decode()))
 
     def __del__(self):
         try:

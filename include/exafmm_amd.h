@@ -110,6 +110,27 @@ int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q);
  * what it needs; the plan stays owned by the caller). */
 int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const double *q);
 
+/* Drop-in path for callers that build their own tree (exafmm-beta's
+ * BuildTree::buildTree, include/build_tree.h:186-209): bodies already in tree
+ * order (x (n,3), q), and the flattened CellBase fields of every cell
+ * (include/types.h:98-113) with children contiguous and each cell's bodies
+ * the range [ibody, ibody+nbody).  The interaction lists are then built on the
+ * host from this tree (Traversal::traverse rules, include/traversal.h:90-177).
+ * Structural errors (ranges, child blocks) return FMM_ERR_ARG. */
+int fmm_set_tree(fmm_ctx *c, int n, const double *x, const double *q, int ncell,
+                 const int *iparent, const int *ichild, const int *nchild, const int *ibody,
+                 const int *nbody, const double *X, const double *R);
+/* Host-only half of fmm_set_tree: validate a caller-built tree (same
+ * arguments; x in tree order, used for the bounds only) and build its
+ * interaction lists for theta and nspawn into a plan that fmm_set_bodies_plan
+ * accepts (with x, q in tree order: the plan's permutation is the identity).
+ * On FMM_ERR_ARG the reason is written to why (NUL-terminated, at most
+ * why_len bytes) when why is not NULL. */
+int fmm_plan_from_tree(fmm_plan **out, int n, const double *x, int ncell, const int *iparent,
+                       const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+                       const double *X, const double *R, double theta, int nspawn, char *why,
+                       int why_len);
+
 /* Re-upload charges only (same positions, same tree and lists). */
 int fmm_set_charges(fmm_ctx *c, const double *q);
 

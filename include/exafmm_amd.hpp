@@ -1,0 +1,536 @@
+// exafmm_amd.hpp -- C++ drop-in for exafmm-beta's Laplace FMM call surface,
+// header-only, on top of the C ABI in exafmm_amd.h (libexafmm_amd.so).
+//
+// Same class names, constructor arguments and call order as the reference's
+// examples/fmm.cxx:13-140:
+//   BoundBox::getBounds        include/bound_box.h:11-25
+//   BuildTree::buildTree       include/build_tree.h:186-209  (permutes bodies in place)
+//   Kernel(P, eps2, wavek)     kernels/laplace.h:116         (owns the device context)
+//   UpDownPass::upwardPass     include/up_down_pass.h:49-60  (P2M + M2M on the GPU)
+//   Traversal::traverse        include/traversal.h:326-352   (lists on the host, M2L + P2P on the GPU)
+//   UpDownPass::downwardPass   include/up_down_pass.h:63-75  (L2L + L2P; TRG += results)
+//   Traversal::direct          include/traversal.h:355-377
+//   Dataset, Verify, Args      include/dataset.h, verify.h, args.h
+// Body and Cell keep the reference's fields; Cell::M and Cell::L are filled
+// after the passes (reference normalisation).  Errors that the reference
+// would hit as undefined behaviour or abort() are reported by throwing
+// exafmm_amd::Error.  Per-pair Kernel::P2P/M2L/... calls are not provided:
+// the passes replace them (one device launch per phase).
+#pragma once
+#include <getopt.h>
+#include <sys/time.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "exafmm_amd.h"
+
+namespace exafmm_amd {
+
+typedef double real_t;
+typedef std::complex<real_t> complex_t;
+const real_t EPS = 1e-16;
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+template <int N> struct vec {
+  real_t d[N];
+  vec() {}
+  vec(real_t v) {
+    for (int i = 0; i < N; i++) d[i] = v;
+  }
+  real_t &operator[](int i) { return d[i]; }
+  const real_t &operator[](int i) const { return d[i]; }
+  vec &operator=(real_t v) {
+    for (int i = 0; i < N; i++) d[i] = v;
+    return *this;
+  }
+};
+typedef vec<3> vec3;
+typedef vec<4> vec4;
+
+//! include/types.h:72-84
+struct Body {
+  vec3 X;
+  real_t SRC;
+  int IBODY;
+  int IRANK;
+  int64_t ICELL;
+  real_t WEIGHT;
+  vec4 TRG;
+};
+typedef std::vector<Body> Bodies;
+typedef Bodies::iterator B_iter;
+
+//! include/types.h:98-119
+struct CellBase {
+  int IPARENT, ICHILD, NCHILD, IBODY, NBODY;
+  uint64_t ICELL;
+  real_t WEIGHT;
+  vec3 X;
+  real_t R;
+  B_iter BODY;
+};
+struct Cell : public CellBase {
+  std::vector<complex_t> M, L;
+};
+typedef std::vector<Cell> Cells;
+typedef Cells::iterator C_iter;
+
+struct Box {
+  vec3 X;
+  real_t R;
+};
+struct Bounds {
+  vec3 Xmin, Xmax;
+};
+
+namespace logger {  // include/logger.h timers (wall clock, named)
+inline bool verbose = false;
+inline const int stringLength = 20;
+inline const int decimal = 7;
+inline std::map<std::string, double> &timers() {
+  static std::map<std::string, double> t;
+  return t;
+}
+inline double now() {
+  timeval tv;
+  gettimeofday(&tv, nullptr);
+  return tv.tv_sec + tv.tv_usec * 1e-6;
+}
+inline void startTimer(const std::string &n) { timers()[n] = now(); }
+inline double stopTimer(const std::string &n) {
+  const double t = now() - timers()[n];
+  if (verbose)
+    std::cout << std::setw(stringLength) << std::left << n << " : " << std::setprecision(decimal)
+              << std::fixed << t << " s" << std::endl;
+  return t;
+}
+inline void printTitle(const std::string &title) {
+  if (!verbose) return;
+  std::string t = "--- " + title + " ";
+  t.resize(34, '-');
+  std::cout << t << std::endl;
+}
+}  // namespace logger
+
+//! include/bound_box.h:11-25 (the 1e-5 per-body drift is kept)
+class BoundBox {
+ public:
+  Bounds getBounds(Bodies &bodies) {
+    Bounds b;
+    if (bodies.empty()) {
+      b.Xmin = b.Xmax = 0.0;
+      return b;
+    }
+    b.Xmin = b.Xmax = bodies.front().X;
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++)
+      for (int d = 0; d < 3; d++) {
+        const real_t lo = b.Xmin[d] - 1e-5, hi = b.Xmax[d] + 1e-5;
+        b.Xmin[d] = B->X[d] < lo ? B->X[d] : lo;
+        b.Xmax[d] = B->X[d] > hi ? B->X[d] : hi;
+      }
+    return b;
+  }
+};
+
+//! include/build_tree.h:186-209: the octree of the reference (same cells,
+//! same order), bodies permuted in place into tree order.
+class BuildTree {
+  const int ncrit;
+  int numLevels = 0;
+
+ public:
+  explicit BuildTree(int _ncrit) : ncrit(_ncrit) {}
+  Cells buildTree(Bodies &bodies, Bodies &buffer, Bounds) {
+    logger::startTimer("Grow tree");
+    const int n = (int)bodies.size();
+    std::vector<double> x(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = bodies[i].X[d];
+    fmm_plan *pl = nullptr;
+    // theta/nspawn only shape the plan's lists, which are not used here
+    if (fmm_plan_build(&pl, n, x.data(), ncrit, 0.4, 5000) != FMM_OK) throw Error("buildTree failed");
+    fmm_stats s;
+    fmm_plan_stats(pl, &s);
+    const int nc = s.ncell;
+    std::vector<int> perm(n), ip(nc), ic(nc), nch(nc), ib(nc), nb(nc);
+    std::vector<double> X(3 * (size_t)nc), R(nc);
+    fmm_plan_tree(pl, perm.data(), ip.data(), ic.data(), nch.data(), ib.data(), nb.data(), X.data(), R.data());
+    fmm_plan_destroy(pl);
+    buffer.resize(n);
+    for (int i = 0; i < n; i++) buffer[i] = bodies[perm[i]];
+    bodies.swap(buffer);
+    logger::stopTimer("Grow tree");
+    Cells cells(nc);
+    for (int c = 0; c < nc; c++) {
+      Cell &C = cells[c];
+      C.IPARENT = ip[c];
+      C.ICHILD = ic[c];
+      C.NCHILD = nch[c];
+      C.IBODY = ib[c];
+      C.NBODY = nb[c];
+      C.ICELL = 0;
+      C.WEIGHT = 0;
+      for (int d = 0; d < 3; d++) C.X[d] = X[3 * (size_t)c + d];
+      C.R = R[c];
+      C.BODY = bodies.begin() + C.IBODY;
+    }
+    numLevels = s.depth;
+    return cells;
+  }
+  void printTreeData(Cells &cells) {
+    if (!logger::verbose || cells.empty()) return;
+    logger::printTitle("Tree stats");
+    std::cout << std::setw(logger::stringLength) << std::left << "Bodies"
+              << " : " << cells.front().NBODY << std::endl
+              << std::setw(logger::stringLength) << std::left << "Cells"
+              << " : " << cells.size() << std::endl
+              << std::setw(logger::stringLength) << std::left << "Tree depth"
+              << " : " << numLevels << std::endl;
+  }
+};
+
+//! kernels/laplace.h:116: owns the device context shared by UpDownPass and
+//! Traversal (the reference shares its Kernel between them the same way).
+class Kernel {
+ public:
+  const int P;
+  const int NTERM;
+  real_t eps2;
+  complex_t wavek;
+  vec3 Xperiodic;
+  fmm_ctx *ctx = nullptr;
+  real_t theta = 0.4;
+  int nspawn = 5000;
+  Cells *uploaded = nullptr;  // cells whose tree is on the device
+
+  Kernel(int _P, real_t _eps2, complex_t _wavek, int device = 0)
+      : P(_P), NTERM(_P * (_P + 1) / 2), eps2(_eps2), wavek(_wavek), Xperiodic(0.0), dev(device) {}
+  ~Kernel() {
+    if (ctx) fmm_destroy(ctx);
+  }
+  Kernel(const Kernel &) = delete;
+  Kernel &operator=(const Kernel &) = delete;
+
+  //! (re)create the context with the traversal parameters
+  void configure(real_t _theta, int _nspawn) {
+    theta = _theta;
+    nspawn = _nspawn;
+    if (ctx) fmm_destroy(ctx);
+    ctx = nullptr;
+    fmm_params p;
+    fmm_default_params(&p);
+    p.P = P;
+    p.eps2 = eps2;
+    p.theta = theta;
+    p.nspawn = nspawn;
+    p.device = dev;
+    const int rc = fmm_create(&ctx, &p);
+    if (rc != FMM_OK) {
+      const std::string m = ctx ? fmm_last_error(ctx) : "fmm_create failed";
+      if (ctx) fmm_destroy(ctx);
+      ctx = nullptr;
+      throw Error(m);
+    }
+  }
+  void check(int rc, const char *what) const {
+    if (rc != FMM_OK) throw Error(std::string(what) + ": " + fmm_last_error(ctx));
+  }
+  //! upload the tree held by cells (bodies in tree order, as buildTree leaves them)
+  void upload(Cells &cells) {
+    if (!ctx) configure(theta, nspawn);
+    const int nc = (int)cells.size();
+    const int n = nc ? cells[0].NBODY : 0;
+    B_iter B0 = nc ? cells[0].BODY : B_iter();
+    std::vector<double> x(3 * (size_t)n), q(n), X(3 * (size_t)nc), R(nc);
+    std::vector<int> ip(nc), ic(nc), nch(nc), ib(nc), nb(nc);
+    for (int i = 0; i < n; i++) {
+      for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = B0[i].X[d];
+      q[i] = B0[i].SRC;
+    }
+    for (int c = 0; c < nc; c++) {
+      ip[c] = cells[c].IPARENT;
+      ic[c] = cells[c].ICHILD;
+      nch[c] = cells[c].NCHILD;
+      ib[c] = cells[c].IBODY;
+      nb[c] = cells[c].NBODY;
+      for (int d = 0; d < 3; d++) X[3 * (size_t)c + d] = cells[c].X[d];
+      R[c] = cells[c].R;
+    }
+    check(fmm_set_tree(ctx, n, x.data(), q.data(), nc, ip.data(), ic.data(), nch.data(), ib.data(),
+                       nb.data(), X.data(), R.data()),
+          "fmm_set_tree");
+    uploaded = &cells;
+  }
+
+ private:
+  int dev;
+};
+
+//! include/up_down_pass.h:46-99
+class UpDownPass {
+  Kernel &kernel;
+
+ public:
+  explicit UpDownPass(Kernel &_kernel) : kernel(_kernel) {}
+  void upwardPass(Cells &cells) {
+    logger::startTimer("Upward pass");
+    if (!cells.empty()) {
+      kernel.upload(cells);
+      kernel.check(fmm_upward(kernel.ctx), "fmm_upward");
+      std::vector<complex_t> M(cells.size() * (size_t)kernel.NTERM);
+      kernel.check(fmm_get_expansions(kernel.ctx, reinterpret_cast<double *>(M.data()), nullptr),
+                   "fmm_get_expansions");
+      for (size_t c = 0; c < cells.size(); c++) {
+        cells[c].M.assign(M.begin() + c * kernel.NTERM, M.begin() + (c + 1) * kernel.NTERM);
+        cells[c].L.assign(kernel.NTERM, 0.0);
+      }
+    }
+    logger::stopTimer("Upward pass");
+  }
+  void downwardPass(Cells &cells) {
+    logger::startTimer("Downward pass");
+    if (!cells.empty()) {
+      if (kernel.uploaded != &cells) throw Error("downwardPass: cells differ from the upward pass");
+      kernel.check(fmm_downward(kernel.ctx), "fmm_downward");
+      const int n = cells[0].NBODY;
+      std::vector<double> trg(4 * (size_t)n);
+      kernel.check(fmm_get_targets(kernel.ctx, trg.data(), 0), "fmm_get_targets");
+      B_iter B0 = cells[0].BODY;
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) B0[i].TRG[k] += trg[4 * (size_t)i + k];
+      std::vector<complex_t> L(cells.size() * (size_t)kernel.NTERM);
+      kernel.check(fmm_get_expansions(kernel.ctx, nullptr, reinterpret_cast<double *>(L.data())),
+                   "fmm_get_expansions");
+      for (size_t c = 0; c < cells.size(); c++)
+        cells[c].L.assign(L.begin() + c * kernel.NTERM, L.begin() + (c + 1) * kernel.NTERM);
+    }
+    logger::stopTimer("Downward pass");
+  }
+};
+
+//! include/traversal.h:291-377 (images = 0, icells == jcells)
+class Traversal {
+  Kernel &kernel;
+  const real_t theta;
+  const int nspawn;
+  const int images;
+
+ public:
+  Traversal(Kernel &_kernel, real_t _theta, int _nspawn, int _images, const char * = "./")
+      : kernel(_kernel), theta(_theta), nspawn(_nspawn), images(_images) {
+    if (images != 0) throw Error("periodic images are not supported (images must be 0)");
+    kernel.configure(theta, nspawn);
+  }
+  void initListCount(Cells &) {}
+  void initWeight(Cells &) {}
+  void traverse(Cells &icells, Cells &jcells, vec3, bool, real_t = 1) {
+    if (icells.empty() || jcells.empty()) return;
+    if (&icells != &jcells) throw Error("traverse: separate target and source trees are not supported");
+    logger::startTimer("Traverse");
+    if (kernel.uploaded != &icells) kernel.upload(icells);
+    kernel.check(fmm_p2p(kernel.ctx), "fmm_p2p");
+    kernel.check(fmm_m2l(kernel.ctx), "fmm_m2l");
+    kernel.check(fmm_synchronize(kernel.ctx), "fmm_synchronize");
+    logger::stopTimer("Traverse");
+  }
+  //! direct sum of jbodies at the positions of ibodies (TRG += ), on the device
+  void direct(Bodies &ibodies, Bodies &jbodies, vec3) {
+    const int nj = (int)jbodies.size(), ni = (int)ibodies.size();
+    if (!kernel.ctx) kernel.configure(theta, nspawn);
+    std::vector<double> xj(3 * (size_t)nj), qj(nj), xi(3 * (size_t)ni), trg(4 * (size_t)ni);
+    for (int j = 0; j < nj; j++) {
+      for (int d = 0; d < 3; d++) xj[3 * (size_t)j + d] = jbodies[j].X[d];
+      qj[j] = jbodies[j].SRC;
+    }
+    for (int i = 0; i < ni; i++)
+      for (int d = 0; d < 3; d++) xi[3 * (size_t)i + d] = ibodies[i].X[d];
+    // the context's bodies become jbodies (a one-leaf tree): only the direct kernel runs
+    const int ip = 0, ic = 0, nch = 0, ib = 0;
+    double X[3] = {0, 0, 0}, R = 1.0;
+    kernel.check(fmm_set_tree(kernel.ctx, nj, xj.data(), qj.data(), nj ? 1 : 0, &ip, &ic, &nch, &ib, &nj,
+                              X, &R),
+                 "fmm_set_tree");
+    kernel.uploaded = nullptr;
+    kernel.check(fmm_direct(kernel.ctx, ni, xi.data(), trg.data()), "fmm_direct");
+    for (int i = 0; i < ni; i++)
+      for (int k = 0; k < 4; k++) ibodies[i].TRG[k] += trg[4 * (size_t)i + k];
+  }
+  void printTraversalData() {
+    if (!logger::verbose || !kernel.ctx) return;
+    fmm_stats s;
+    fmm_get_stats(kernel.ctx, &s);
+    std::cout << "--- Traversal stats --------------" << std::endl
+              << std::setw(logger::stringLength) << std::left << "P2P calls"
+              << " : " << s.p2p_pairs << std::endl
+              << std::setw(logger::stringLength) << std::left << "M2L calls"
+              << " : " << s.m2l_pairs << std::endl;
+  }
+  void writeList(Cells, int) {}
+};
+
+//! include/dataset.h: seeded drand48 generators in the reference's draw order
+class Dataset {
+ public:
+  Bodies initBodies(int numBodies, const char *distribution, int mpirank = 0, int = 1, int = 1) {
+    std::vector<double> x(3 * (size_t)numBodies), q(numBodies);
+    if (fmm_make_bodies(numBodies, distribution, mpirank, x.data(), q.data()) != FMM_OK)
+      throw Error(std::string("Unknown data distribution ") + distribution);
+    Bodies bodies(numBodies);
+    for (int i = 0; i < numBodies; i++) {
+      for (int d = 0; d < 3; d++) bodies[i].X[d] = x[3 * (size_t)i + d];
+      bodies[i].SRC = q[i];
+      bodies[i].IRANK = mpirank;
+    }
+    initTarget(bodies);
+    return bodies;
+  }
+  void initTarget(Bodies &bodies) {
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++) {
+      B->TRG = 0.0;
+      B->IBODY = (int)(B - bodies.begin());
+      B->ICELL = 0;
+      B->WEIGHT = 1;
+    }
+  }
+  void sampleBodies(Bodies &bodies, int numTargets) {
+    if (numTargets < (int)bodies.size()) {
+      const int stride = (int)bodies.size() / numTargets;
+      for (int i = 0; i < numTargets; i++) bodies[i] = bodies[i * stride];
+      bodies.resize(numTargets);
+    }
+  }
+};
+
+//! include/verify.h:42-92, 97-106
+class Verify {
+ public:
+  bool verbose = false;
+  double getNrmScalar(Bodies &b) {
+    double v = 0;
+    for (B_iter B = b.begin(); B != b.end(); B++) v += std::abs(B->TRG[0] * B->TRG[0]);
+    return v;
+  }
+  double getDifScalar(Bodies &b, Bodies &b2) {
+    double v = 0;
+    for (size_t i = 0; i < b.size(); i++) v += std::abs((b[i].TRG[0] - b2[i].TRG[0]) * (b[i].TRG[0] - b2[i].TRG[0]));
+    return v;
+  }
+  double getNrmVector(Bodies &b) {
+    double v = 0;
+    for (B_iter B = b.begin(); B != b.end(); B++)
+      v += std::abs(B->TRG[1] * B->TRG[1] + B->TRG[2] * B->TRG[2] + B->TRG[3] * B->TRG[3]);
+    return v;
+  }
+  double getDifVector(Bodies &b, Bodies &b2) {
+    double v = 0;
+    for (size_t i = 0; i < b.size(); i++) {
+      double s = 0;
+      for (int d = 1; d < 4; d++) s += (b[i].TRG[d] - b2[i].TRG[d]) * (b[i].TRG[d] - b2[i].TRG[d]);
+      v += std::abs(s);
+    }
+    return v;
+  }
+  void print(const std::string &title, double v) {
+    if (verbose)
+      std::cout << std::setw(logger::stringLength) << std::left << title << " : "
+                << std::setprecision(logger::decimal) << std::scientific << v << std::endl;
+  }
+};
+
+//! include/args.h:15-35, 188-274: same options and defaults
+class Args {
+ public:
+  int accuracy = 0, ncrit = 64, dual = 0, graft = 0, getMatrix = 0, images = 0, IneJ = 0;
+  int numBodies = 1000000, P = 10, repeat = 1, nspawn = 5000, threads = 16, verbose = 0, write = 0;
+  double cutoff = 0.0, theta = 0.4;
+  const char *distribution = "cube";
+  const char *path = "./";
+
+  Args(int argc = 0, char **argv = nullptr) {
+    static struct option opts[] = {
+        {"accuracy", no_argument, 0, 'a'},     {"ncrit", required_argument, 0, 'c'},
+        {"cutoff", required_argument, 0, 'C'}, {"distribution", required_argument, 0, 'd'},
+        {"dual", no_argument, 0, 'D'},         {"graft", no_argument, 0, 'g'},
+        {"getMatrix", no_argument, 0, 'G'},    {"help", no_argument, 0, 'h'},
+        {"images", required_argument, 0, 'i'}, {"IneJ", no_argument, 0, 'j'},
+        {"numBodies", required_argument, 0, 'n'}, {"path", required_argument, 0, 'p'},
+        {"P", required_argument, 0, 'P'},      {"repeat", required_argument, 0, 'r'},
+        {"nspawn", required_argument, 0, 's'}, {"theta", required_argument, 0, 't'},
+        {"threads", required_argument, 0, 'T'}, {"verbose", no_argument, 0, 'v'},
+        {"write", no_argument, 0, 'w'},        {0, 0, 0, 0}};
+    optind = 1;
+    while (argc > 0) {
+      int idx;
+      const int c = getopt_long(argc, argv, "ac:C:d:DgGhi:jn:p:P:r:s:t:T:vw", opts, &idx);
+      if (c == -1) break;
+      switch (c) {
+        case 'a': accuracy = 1; break;
+        case 'c': ncrit = atoi(optarg); break;
+        case 'C': cutoff = atof(optarg); break;
+        case 'd': distribution = parseDistribution(optarg); break;
+        case 'D': dual = 1; break;
+        case 'g': graft = 1; break;
+        case 'G': getMatrix = 1; break;
+        case 'i': images = atoi(optarg); break;
+        case 'j': IneJ = 1; break;
+        case 'n': numBodies = atoi(optarg); break;
+        case 'p': path = optarg; break;
+        case 'P': P = atoi(optarg); break;
+        case 'r': repeat = atoi(optarg); break;
+        case 's': nspawn = atoi(optarg); break;
+        case 't': theta = atof(optarg); break;
+        case 'T': threads = atoi(optarg); break;
+        case 'v': verbose = 1; break;
+        case 'w': write = 1; break;
+        default: usage(argv[0]); throw Error("bad command line");
+      }
+    }
+  }
+  static const char *parseDistribution(const char *arg) {
+    switch (arg[0]) {
+      case 'c': return arg[1] == '1' ? "cube1" : "cube";
+      case 'p': return arg[strlen_c(arg) - 1] == '1' ? "plummer1" : "plummer";
+      case 's': return "sphere";
+      case 'o': return "octant";
+      default: throw Error(std::string("invalid distribution ") + arg);
+    }
+  }
+  static size_t strlen_c(const char *s) {
+    size_t n = 0;
+    while (s[n]) n++;
+    return n;
+  }
+  static void usage(const char *name) {
+    std::fprintf(stderr,
+                 "Usage: %s [options]\n"
+                 " -a accuracy only  -c ncrit (64)  -d cube|sphere|octant|plummer|cube1|plummer1\n"
+                 " -n numBodies (1000000)  -P order (10)  -r repeat (1)  -s nspawn (5000)\n"
+                 " -t theta (0.4)  -v verbose\n",
+                 name);
+  }
+  void print() const {
+    if (!verbose) return;
+    const int w = logger::stringLength;
+    std::cout << std::setw(w) << std::left << "numBodies" << " : " << numBodies << std::endl
+              << std::setw(w) << std::left << "ncrit" << " : " << ncrit << std::endl
+              << std::setw(w) << std::left << "distribution" << " : " << distribution << std::endl
+              << std::setw(w) << std::left << "P" << " : " << P << std::endl
+              << std::setw(w) << std::left << "theta" << " : " << theta << std::endl
+              << std::setw(w) << std::left << "nspawn" << " : " << nspawn << std::endl;
+  }
+};
+
+}  // namespace exafmm_amd

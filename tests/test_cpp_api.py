@@ -1,0 +1,159 @@
+"""The C++ drop-in layer (include/exafmm_amd.hpp) used the way the reference's
+examples/fmm.cxx uses its classes.
+
+CPU: a host-only program (tests/cpp/tree_dump.cpp: Args, Dataset, BoundBox,
+BuildTree) is compiled with g++ and its bodies and cells are compared with the
+oracle's generator and tree.  GPU: the example driver (examples/fmm.cpp)
+prints the same "Rel. L2 Error" lines as the reference driver, checked against
+the reference's own printed values; fmm_set_tree with a caller-built tree
+gives the same result as fmm_set_bodies.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "exafmm-beta_amd")
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+@pytest.fixture(scope="module")
+def tree_dump(tmp_path_factory):
+    F = _F()
+    exe = str(tmp_path_factory.mktemp("cpp") / "tree_dump")
+    libdir = os.path.dirname(F.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "tree_dump.cpp"), "-o", exe,
+                    "-L", libdir, "-lexafmm_amd", "-Wl,-rpath," + libdir], check=True)
+    return exe
+
+
+def _run_dump(exe, *args):
+    out = subprocess.run([exe, *args], check=True, capture_output=True, text=True).stdout.split("\n")
+    head = out[0].split()
+    lo_hi = np.array(out[1].split(), float)
+    nb, nc = map(int, out[2].split())
+    bodies = np.array([l.split() for l in out[3:3 + nb]], float).reshape(nb, 5)
+    cells = np.array([l.split() for l in out[3 + nb:3 + nb + nc]], float).reshape(nc, 10)
+    return head, lo_hi, bodies, cells
+
+
+@pytest.mark.parametrize("argv,n,ncrit,dist", [
+    ([], 1000000, 64, "cube"),
+    (["-n", "5000", "-c", "16", "-d", "plummer"], 5000, 16, "plummer"),
+    (["--numBodies", "3000", "--ncrit", "8", "--distribution", "c"], 3000, 8, "cube"),
+    (["-n", "1"], 1, 64, "cube")])
+def test_cpp_tree_matches_oracle(tree_dump, argv, n, ncrit, dist):
+    if not argv:  # the defaults only: parse, do not build a 1M tree here
+        argv = ["-n", "2000"]
+        n = 2000
+    head, lo_hi, b, c = _run_dump(tree_dump, *argv)
+    assert int(head[0]) == n and int(head[1]) == ncrit and head[4] == dist
+    assert int(head[2]) == 10 and float(head[3]) == 0.4  # Args defaults (include/args.h)
+    x = O.data_cube(n, 0) if dist == "cube" else O.data_plummer(n, 0)
+    q = O.data_source(n, 0)
+    lo, hi = O.bounds(x)
+    assert np.array_equal(lo_hi, np.concatenate([lo, hi]))
+    t = O.Tree(x, ncrit)
+    assert np.array_equal(b[:, 0].astype(int), t.perm)  # IBODY keeps the input index
+    assert np.array_equal(b[:, 1:4], x[t.perm]) and np.array_equal(b[:, 4], q[t.perm])
+    assert len(c) == t.ncell
+    for k, name in enumerate(("iparent", "ichild", "nchild", "ibody", "nbody")):
+        assert np.array_equal(c[:, k].astype(int), getattr(t, name)), name
+    assert np.array_equal(c[:, 5:8], t.X) and np.array_equal(c[:, 8], t.R)
+    assert np.array_equal(c[:, 9].astype(int), t.ibody)  # Cell::BODY points at IBODY
+
+
+def test_cpp_args_rejects_bad_distribution(tree_dump):
+    r = subprocess.run([tree_dump, "-n", "10", "-d", "x"], capture_output=True, text=True)
+    assert r.returncode != 0
+
+
+@pytest.mark.gpu
+def test_example_driver_prints_reference_errors():
+    """examples/fmm.cxx defaults on 32768 cube bodies print
+    4.7429586e-06 / 4.8780842e-06 (the reference's output)."""
+    exe = os.path.join(PKG, "lib", "fmm")
+    out = subprocess.run([exe, "-n", "32768", "-v"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = dict(re.findall(r"Rel\. L2 Error \((pot|acc)\)\s*:\s*(\S+)", out.stdout))
+    assert got == {"pot": "4.7429586e-06", "acc": "4.8780842e-06"}, out.stdout
+    assert re.search(r"Cells\s*:\s*\d+", out.stdout) and re.search(r"M2L calls\s*:\s*\d+", out.stdout)
+
+
+@pytest.mark.gpu
+def test_set_tree_equals_set_bodies():
+    F = _F()
+    x = O.data_plummer(6000, 4)
+    q = O.data_source(6000, 4)
+    t = O.Tree(x, 32)
+    tree = {"iparent": t.iparent, "ichild": t.ichild, "nchild": t.nchild, "ibody": t.ibody,
+            "nbody": t.nbody, "X": t.X, "R": t.R}
+    out = []
+    for use_tree in (False, True):
+        f = F.FMM(P=8, ncrit=32, m2l_mode=F.M2L_TARGET)
+        if use_tree:
+            f.set_tree(x[t.perm], q[t.perm], tree)
+        else:
+            f.set_bodies(x, q)
+        f.evaluate()
+        out.append((f.targets(input_order=False), f.expansions(), f.stats()))
+        f.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1][0], out[1][1][0]) and np.array_equal(out[0][1][1], out[1][1][1])
+    assert out[0][2]["m2l_pairs"] == out[1][2]["m2l_pairs"]
+
+
+def _tree_dict(t, **over):
+    d = {"iparent": t.iparent, "ichild": t.ichild, "nchild": t.nchild, "ibody": t.ibody,
+         "nbody": t.nbody, "X": t.X, "R": t.R}
+    d.update({k: v for k, v in over.items()})
+    return d
+
+
+@pytest.mark.parametrize("theta,nspawn", [(0.4, 5000), (0.3, 200)])
+def test_plan_from_tree_lists_match_oracle(theta, nspawn):
+    F = _F()
+    x = O.data_plummer(8000, 6)
+    t = O.Tree(x, 16)
+    pl = F.Plan(x[t.perm], theta=theta, nspawn=nspawn, tree=_tree_dict(t))
+    tr = pl.tree()
+    assert np.array_equal(tr["perm"], np.arange(len(x))) and tr["depth"] == t.depth
+    ci, cj, ty = t.traverse(theta, nspawn)
+    mp, ms, pp, ps = pl.lists()
+    for typ, ptr, srcs in ((0, mp, ms), (1, pp, ps)):
+        sel = ty == typ
+        a, b = ci[sel], cj[sel]
+        order = np.argsort(a, kind="stable")
+        optr = np.zeros(t.ncell + 1, np.int64)
+        optr[1:] = np.cumsum(np.bincount(a, minlength=t.ncell))
+        assert np.array_equal(ptr, optr) and np.array_equal(srcs, b[order])
+
+
+def test_plan_from_tree_rejects_inconsistent_trees():
+    F = _F()
+    x = O.data_cube(500, 1)
+    t = O.Tree(x, 64)
+    xs = x[t.perm]
+    leaf = int(np.nonzero(t.nchild == 0)[0][-1])
+    ib = t.ibody.copy()
+    ib[leaf] += 1                       # leaves no longer partition the bodies
+    nb = t.nbody.copy()
+    nb[0] -= 1                          # root does not cover every body
+    ip = t.iparent.copy()
+    ip[leaf] = leaf                     # parent link broken
+    ic = t.ichild.copy()
+    ic[0] = 0                           # child block does not follow the parent
+    for bad in (dict(ibody=ib), dict(nbody=nb), dict(iparent=ip), dict(ichild=ic)):
+        with pytest.raises(F.FMMError, match="fmm_plan_from_tree"):
+            F.Plan(xs, tree=_tree_dict(t, **bad))
+    F.Plan(xs, tree=_tree_dict(t))      # the unmodified tree is accepted
