@@ -9,9 +9,13 @@ dominant kernel and a CPU baseline (exafmm-beta's own operators, one core, on
 a bounded sample of the same lists).
 
 Default workload: N = 2^20 Plummer bodies (unit scale radius, r < 10,
-q = 1/N), P = 10, theta = 0.4, ncrit = 64, FP64.
+q = 1/N), P = 10, theta = 0.4, ncrit = 64, FP64.  The *-mixed workloads
+(N = 2^23 uniform cube, P = 8 / 12) run the mixed-precision solve and the FP64
+solve of the same system, and report both with their errors against the FP64
+direct sum.
 
   python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+  python bench.py --copy-probe MIB   # one device copy of MIB MiB (HBM counter calibration)
 """
 import argparse
 import json
@@ -27,18 +31,68 @@ sys.path.insert(0, os.path.join(ROOT, "exafmm-beta_amd"))
 
 METRIC = "particles/s per FMM solve + p50/p99 latency, mixed vs FP64 at equal L2 error"
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 matrix (MFMA)
+PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X FP32 vector (the packed-FP32 rate)
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
     # N = 2^20 Plummer, FP64, P = 10: the single-GPU FP64 solve
     "plummer1m-p10-fp64": dict(n=1 << 20, dist="plummer1", P=10, theta=0.4, ncrit=64, nspawn=5000,
-                               ntarget=1000,
+                               ntarget=1000, precision="fp64",
                                desc="N=2^20 Plummer (a=1, r<10), q=1/N, P=10, theta=0.4, ncrit=64"),
     # N = 32768 uniform cube [-1,1]^3, zero-mean q in [-1,1]: CPU-runnable case
     "cube32k-p10-fp64": dict(n=32768, dist="cube1", P=10, theta=0.4, ncrit=64, nspawn=5000,
-                             ntarget=100,
+                             ntarget=100, precision="fp64",
                              desc="N=32768 cube [-1,1]^3, zero-mean q in [-1,1], P=10, theta=0.4"),
+    # N = 2^23 uniform cube, mixed precision, with the FP64 solve of the same
+    # system measured beside it (same tree and lists)
+    "cube8m-p8-mixed": dict(n=8388608, dist="cube1", P=8, theta=0.4, ncrit=64, nspawn=5000,
+                            ntarget=1000, precision="mixed",
+                            desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=8, theta=0.4, "
+                                 "ncrit=64; FP32 P2P/M2L, FP64 P2M/M2M/L2L/L2P"),
+    "cube8m-p12-mixed": dict(n=8388608, dist="cube1", P=12, theta=0.4, ncrit=64, nspawn=5000,
+                             ntarget=1000, precision="mixed",
+                             desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=12, theta=0.4, "
+                                  "ncrit=64; FP32 P2P/M2L, FP64 P2M/M2M/L2L/L2P"),
 }
+
+# Serving stream: independent systems, N = floor(2^U[12,18]), uniform cube or
+# Plummer with equal probability, zero-mean uniform charges, P = 8, theta =
+# 0.4, mixed precision, Poisson arrivals at `rate` requests/s.
+SERVE = {
+    "serve-200": dict(rate=200.0, requests=2048, P=8, theta=0.4, ncrit=64, nspawn=5000,
+                      precision="mixed", log2n=(12, 18)),
+    "serve-1000": dict(rate=1000.0, requests=2048, P=8, theta=0.4, ncrit=64, nspawn=5000,
+                       precision="mixed", log2n=(12, 18)),
+}
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB through a copy kernel: the known
+    byte count that FETCH_SIZE / WRITE_SIZE of a profiled run are scaled by."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("[copy-probe] copied %d MiB (read %d B, write %d B)" % (mib, 4 * n, 4 * n), flush=True)
+
+
+def traffic_from_profile(workload, kernel):
+    """HBM bytes per launch of `kernel` for `workload`, from the committed
+    rocprofv3 --pmc summary (profiles/<kernel>_<workload>_traffic.md, a line
+    'bytes_per_launch: <fetch> <write>'), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, workload))
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_launch:"):
+                fetch, write = (float(v) for v in line.split(":", 1)[1].split())
+                return fetch + write
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def m2l_flops(P, pairs):
@@ -114,38 +168,19 @@ def cpu_baseline(x, q, w, budget_s):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plummer1m-p10-fp64", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build evaluates one system on one GPU; multi-GPU runs are not "
-              "supported (use --gpus 1)", file=sys.stderr)
-        sys.exit(2)
-
-    import exafmm_amd as F
-    w = WORKLOADS[args.workload]
+def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log):
+    """Upload one system (host plan shared), run `warmup` + `steps` solves, and
+    return timings, kernel figures and the accuracy on sampled targets."""
     n, P = w["n"], w["P"]
-    x, q = F.make_bodies(n, w["dist"], 0)
-    f = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"])
-    t_set = time.perf_counter()
-    f.set_bodies(x, q)
-    t_set = time.perf_counter() - t_set
-    st0 = f.stats()
-    print("[bench] %s: ncell=%d depth=%d m2l=%d p2p=%d body_pairs=%d; host setup %.2f s"
-          % (args.workload, st0["ncell"], st0["depth"], st0["m2l_pairs"], st0["p2p_pairs"],
-             st0["p2p_body_pairs"], t_set), file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    f = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"],
+              precision=F.FMM_MIXED if precision == "mixed" else F.FMM_FP64)
+    f.set_bodies_plan(plan, q)
+    for _ in range(warmup):
         f.evaluate()
     f.synchronize()
     lat, dev, km, kp = [], [], [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         a = time.perf_counter()
         f.evaluate()           # returns after the last kernel completes
         lat.append(time.perf_counter() - a)
@@ -154,39 +189,163 @@ def main():
         km.append(s["dev_m2l_kernel_ms"])
         kp.append(s["dev_p2p_kernel_ms"])
     f.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    value = n * args.steps / wall
+    wall = time.perf_counter() - t0
     st = f.stats()
-    print("[bench] timed %d steps: %.3f ms/step (device %.3f ms: up %.3f p2p %.3f m2l %.3f down %.3f)"
-          % (args.steps, 1e3 * wall / args.steps, np.mean(dev), st["dev_upward_ms"], st["dev_p2p_ms"],
-             st["dev_m2l_ms"], st["dev_downward_ms"]), file=sys.stderr, flush=True)
-
-    # accuracy: Traversal::direct on GPU at sampleBodies-style tree-order targets
+    log("[bench] %s %s: %d steps, %.3f ms/step (device %.3f ms: up %.3f p2p %.3f m2l %.3f down %.3f)"
+        % (w["desc"].split(",")[0], precision, steps, 1e3 * wall / steps, np.mean(dev),
+           st["dev_upward_ms"], st["dev_p2p_ms"], st["dev_m2l_ms"], st["dev_downward_ms"]))
+    # accuracy: Traversal::direct (FP64, IEEE sqrt/div) on the GPU at
+    # Dataset::sampleBodies-style tree-order targets
     trg = f.targets(input_order=False)
     perm = f.tree()["perm"]
-    stride = n // w["ntarget"]
-    idx = np.arange(w["ntarget"]) * stride
-    xs = x[perm]
-    d = f.direct(xs[idx])
+    idx = np.arange(w["ntarget"]) * (n // w["ntarget"])
+    d = f.direct(x[perm][idx])
     pot_rel, acc_rel = rel_l2(d, trg[idx])
-
+    f.close()
     m2l_ms, p2p_ms = float(np.mean(km)), float(np.mean(kp))
     fm = m2l_flops(P, st["m2l_pairs"])
     fp = p2p_flops(st["p2p_body_pairs"])
-    kernels = {
-        "m2l": {"avg_ms": m2l_ms, "alg_flops": fm, "tflops": fm / m2l_ms * 1e-9,
-                "frac_fp64_peak": fm / m2l_ms * 1e-9 / PEAK_FP64_TFLOPS},
-        "p2p": {"avg_ms": p2p_ms, "alg_flops": fp, "tflops": fp / p2p_ms * 1e-9,
-                "frac_fp64_peak": fp / p2p_ms * 1e-9 / PEAK_FP64_TFLOPS},
+    peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
+    return {
+        "wall_s": wall, "lat": lat, "dev": dev, "st": st, "trg": trg, "perm": perm,
+        "accuracy": {"pot_rel_l2": pot_rel, "acc_rel_l2": acc_rel, "targets": int(w["ntarget"]),
+                     "reference": "FP64 direct sum (Traversal::direct) on the GPU, IEEE sqrt/div"},
+        "kernels": {
+            "m2l": {"avg_ms": m2l_ms, "alg_flops": fm, "tflops": fm / m2l_ms * 1e-9,
+                    "peak_tflops": peak, "frac": fm / m2l_ms * 1e-9 / peak},
+            "p2p": {"avg_ms": p2p_ms, "alg_flops": fp, "tflops": fp / p2p_ms * 1e-9,
+                    "peak_tflops": PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_VALU_TFLOPS,
+                    "frac": fp / p2p_ms * 1e-9 / (PEAK_FP64_TFLOPS if precision == "fp64"
+                                                   else PEAK_FP32_VALU_TFLOPS)},
+        },
     }
-    dom = "m2l" if m2l_ms >= p2p_ms else "p2p"
-    ach = kernels[dom]["tflops"]
-    roofline = {"bound": "mfma", "engine": "fp64 VALU (same 78.6 TF peak as FP64 MFMA)",
-                "kernel": dom, "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
 
+
+def serve_bench(F, name, nreq, check_every, log):
+    """Poisson arrivals into the request loop; latency = submit -> result on
+    the server's clock; throughput over the whole stream; rel-L2 vs the FP64
+    direct sum on 100 sampled targets of every `check_every`-th request."""
+    w = SERVE[name]
+    nreq = nreq or w["requests"]
+    rng = np.random.default_rng(0)
+    lo, hi = w["log2n"]
+    sizes = np.floor(2.0 ** rng.uniform(lo, hi, nreq)).astype(np.int64)
+    kinds = rng.integers(0, 2, nreq)
+    gaps = rng.exponential(1.0 / w["rate"], nreq)
+    systems = [F.make_bodies(int(n), "cube1" if k == 0 else "plummer1", 1000 + i)
+               for i, (n, k) in enumerate(zip(sizes, kinds))]
+    log("[serve] %s: %d requests, %d bodies, mean N %.0f, rate %.0f/s"
+        % (name, nreq, int(sizes.sum()), sizes.mean(), w["rate"]))
+    prec = F.FMM_MIXED if w["precision"] == "mixed" else F.FMM_FP64
+    srv = F.Server(P=w["P"], theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], precision=prec,
+                   slots=3, batch_bodies=1 << 20, batch_requests=64)
+    for x, q in systems[:8]:   # warm-up: contexts allocate, kernels load
+        srv.wait(srv.submit(x, q))
+    base = srv.stats()
+    ids = []
+    t0 = time.perf_counter()
+    due = t0
+    for (x, q), gap in zip(systems, gaps):
+        due += gap
+        d = due - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        ids.append(srv.submit(x, q))
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    infos = [r[1] for r in res]
+    lat = np.array([i["t_done_ms"] - i["t_submit_ms"] for i in infos])
+    span_s = 1e-3 * (max(i["t_done_ms"] for i in infos) - min(i["t_submit_ms"] for i in infos))
+    errs = []
+    for k in range(0, nreq, check_every):
+        x, q = systems[k]
+        f = F.FMM(P=w["P"], theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"])
+        f.set_bodies(x, q)
+        idx = np.arange(100) * (len(x) // 100)
+        d = f.direct(x[idx])
+        f.close()
+        errs.append(rel_l2(d, res[k][0][idx]))
+    errs = np.array(errs)
+    batches = st["batches"] - base["batches"]
+    out = {
+        "value": float(sizes.sum()) / span_s, "unit": "particles/s",
+        "requests_per_s": nreq / span_s, "offered_rate": w["rate"], "requests": nreq,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean()), "max": float(lat.max())},
+        "plan_wait_ms_p50": float(np.percentile([i["t_planned_ms"] - i["t_submit_ms"] for i in infos], 50)),
+        "batches": batches, "mean_batch_requests": nreq / max(batches, 1),
+        "max_batch_requests": st["max_batch_requests"],
+        "device_busy_frac": (st["device_ms"] - base["device_ms"]) / (1e3 * span_s),
+        "accuracy": {"checked_requests": len(errs), "pot_rel_l2_max": float(errs[:, 0].max()),
+                     "acc_rel_l2_max": float(errs[:, 1].max()),
+                     "pot_rel_l2_median": float(np.median(errs[:, 0])),
+                     "acc_rel_l2_median": float(np.median(errs[:, 1])),
+                     "reference": "FP64 direct sum on 100 sampled targets per checked request"},
+        "config": {"workload": name, "P": w["P"], "theta": w["theta"], "ncrit": w["ncrit"],
+                   "precision": w["precision"], "log2n": list(w["log2n"]),
+                   "inputs": "N=floor(2^U[12,18]), cube [-1,1]^3 or Plummer (p=1/2), "
+                             "zero-mean q; Poisson arrivals"},
+    }
+    log("[serve] %.1f req/s, %.3g particles/s, latency p50 %.1f ms p99 %.1f ms, %d batches"
+        % (out["requests_per_s"], out["value"], out["latency_ms"]["p50"], out["latency_ms"]["p99"], batches))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plummer1m-p10-fp64", choices=sorted(WORKLOADS) + sorted(SERVE))
+    ap.add_argument("--requests", type=int, default=0, help="serve-*: number of requests (default 2048)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build evaluates one system on one GPU; multi-GPU runs are not "
+              "supported (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    import exafmm_amd as F
+    if args.workload in SERVE:
+        r = serve_bench(F, args.workload, args.requests, 64, log)
+        out = {"metric": METRIC, "value": r.pop("value"), "unit": r.pop("unit"), "n_gpus": 1,
+               "steps": r["requests"], "warmup": 8, "ms_per_step": r["latency_ms"]["mean"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32+f64" if SERVE[args.workload]["precision"] == "mixed" else "f64",
+               "data": "synthetic", "config": r.pop("config"), **r}
+        print(json.dumps(out))
+        return
+    w = WORKLOADS[args.workload]
+    n, P = w["n"], w["P"]
+    x, q = F.make_bodies(n, w["dist"], 0)
+    t_plan = time.perf_counter()
+    plan = F.Plan(x, ncrit=w["ncrit"], theta=w["theta"], nspawn=w["nspawn"])
+    t_plan = time.perf_counter() - t_plan
+    st0 = plan.stats()
+    log("[bench] %s: ncell=%d depth=%d m2l=%d p2p=%d body_pairs=%d; host plan %.2f s"
+        % (args.workload, st0["ncell"], st0["depth"], st0["m2l_pairs"], st0["p2p_pairs"],
+           st0["p2p_body_pairs"], t_plan))
+    r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log)
+    st = r["st"]
+    kernels = r["kernels"]
+    dom = "m2l" if kernels["m2l"]["avg_ms"] >= kernels["p2p"]["avg_ms"] else "p2p"
+    kd = kernels[dom]
+    # M2L runs on MFMA (grouped path: FP64 MFMA, or FP32 MFMA when mixed)
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": kd["tflops"], "peak": kd["peak_tflops"],
+                "unit": "TFLOP/s", "frac": kd["frac"],
+                "traffic": traffic_from_profile(args.workload, dom)}
     host_ms = st["host_bounds_ms"] + st["host_tree_ms"] + st["host_lists_ms"] + st["host_upload_ms"]
+    value = n * args.steps / r["wall_s"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -194,33 +353,43 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * r["wall_s"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if w["precision"] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "P": P, "theta": w["theta"],
                    "ncrit": w["ncrit"], "nspawn": w["nspawn"], "inputs": w["desc"],
-                   "precision": "fp64"},
-        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
-                       "p99": 1e3 * float(np.percentile(lat, 99))},
-        "device_ms": {"total": float(np.mean(dev)), "upward": st["dev_upward_ms"],
+                   "precision": w["precision"]},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(r["lat"], 50)),
+                       "p99": 1e3 * float(np.percentile(r["lat"], 99))},
+        "device_ms": {"total": float(np.mean(r["dev"])), "upward": st["dev_upward_ms"],
                       "p2p": st["dev_p2p_ms"], "m2l": st["dev_m2l_ms"],
                       "downward": st["dev_downward_ms"]},
         "host_ms": {"bounds": st["host_bounds_ms"], "tree": st["host_tree_ms"],
                     "lists": st["host_lists_ms"], "upload": st["host_upload_ms"]},
-        "end_to_end_particles_per_s": n / (1e-3 * (host_ms + float(np.mean(dev)))),
-        "accuracy": {"pot_rel_l2": pot_rel, "acc_rel_l2": acc_rel, "targets": int(w["ntarget"]),
-                     "reference": "direct sum (Traversal::direct) on the GPU, IEEE sqrt/div"},
+        "end_to_end_particles_per_s": n / (1e-3 * (host_ms + float(np.mean(r["dev"])))),
+        "accuracy": r["accuracy"],
         "counts": {"cells": st["ncell"], "leaves": st["nleaf"], "depth": st["depth"],
                    "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"],
-                   "p2p_body_pairs": st["p2p_body_pairs"]},
+                   "p2p_body_pairs": st["p2p_body_pairs"], "m2l_groups": st["m2l_groups"]},
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    f.close()
+    if w["precision"] == "mixed":
+        # the FP64 solve of the same system, for "mixed vs FP64 at equal L2 error"
+        steps64 = max(1, args.steps // 2)
+        r64 = solve_and_time(F, w, plan, x, q, "fp64", steps64, 1, log)
+        v64 = n * steps64 / r64["wall_s"]
+        out["fp64_same_system"] = {
+            "value": v64, "ms_per_step": 1e3 * r64["wall_s"] / steps64,
+            "latency_ms": {"p50": 1e3 * float(np.percentile(r64["lat"], 50)),
+                           "p99": 1e3 * float(np.percentile(r64["lat"], 99))},
+            "accuracy": r64["accuracy"], "kernels": r64["kernels"]}
+        out["mixed_over_fp64"] = value / v64
+        out["mixed_vs_fp64_rel_l2"] = dict(zip(("pot", "acc"), rel_l2(r64["trg"], r["trg"])))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, q, w, args.cpu_budget)
     print(json.dumps(out))

@@ -60,12 +60,17 @@ struct fmm_ctx {
   int nchunk = 0;
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
+  // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
+  bool mixed = false;
+  DBuf g_sc, cr, M32;
 };
 
 struct fmm_plan {
   int n = 0;
   fmm::Tree tree;
   fmm::Lists lists;
+  fmm::M2LGroups groups;  // prebuilt M2L groups (fmm_plan_build_groups), kind:
+  int groups_kind = -1;   // -1 none, FMM_FP64 offset groups, FMM_MIXED by level
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   double t_bounds = 0, t_tree = 0, t_lists = 0;
 };
@@ -93,7 +98,9 @@ int ensure(fmm_ctx *c, DBuf &b, size_t bytes) {
     b.p = nullptr;
     b.cap = 0;
   }
-  size_t sz = bytes ? bytes : 16;
+  // 25% headroom: a stream of slightly growing systems reallocates rarely
+  // (hipMalloc/hipFree synchronise the device)
+  size_t sz = bytes ? bytes + bytes / 4 : 16;
   HIPCHK(hipMalloc(&b.p, sz));
   b.cap = sz;
   return FMM_OK;
@@ -175,8 +182,12 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
   } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET) {
     c->err = "invalid m2l_mode";
     rc = FMM_ERR_ARG;
-  } else if (p->precision != FMM_FP64) {
+  } else if (p->precision != FMM_FP64 && p->precision != FMM_MIXED) {
     c->err = "unsupported precision mode";
+    rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->precision == FMM_MIXED &&
+             (!fmmdev::m2l_mixed_supported(p->P) || p->m2l_mode != FMM_M2L_AUTO)) {
+    c->err = "mixed precision needs 2 <= P <= 12 and m2l_mode AUTO";
     rc = FMM_ERR_UNSUPPORTED;
   }
   if (rc == FMM_OK) {
@@ -290,6 +301,55 @@ int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double the
 }
 
 void fmm_plan_destroy(fmm_plan *pl) { delete pl; }
+
+int fmm_plan_build_groups(fmm_plan *pl, int precision) {
+  if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED)) return FMM_ERR_ARG;
+  try {
+    pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED);
+  } catch (const std::bad_alloc &) {
+    return FMM_ERR_MEMORY;
+  }
+  pl->groups_kind = precision;
+  return FMM_OK;
+}
+
+int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans) {
+  if (!out || k < 1 || !plans) return FMM_ERR_ARG;
+  *out = nullptr;
+  const int kind = plans[0] ? plans[0]->groups_kind : -1;
+  std::vector<fmm::PlanPart> parts;
+  int64_t n = 0;
+  for (int i = 0; i < k; i++) {
+    if (!plans[i]) return FMM_ERR_ARG;
+    n += plans[i]->n;
+    parts.push_back({&plans[i]->tree, &plans[i]->lists, &plans[i]->groups});
+  }
+  if (n > INT32_MAX) return FMM_ERR_ARG;
+  bool same_kind = true;
+  for (int i = 0; i < k; i++) same_kind = same_kind && plans[i]->groups_kind == kind;
+  fmm_plan *pl = new (std::nothrow) fmm_plan();
+  if (!pl) return FMM_ERR_MEMORY;
+  try {
+    const bool with_groups = same_kind && kind >= 0;
+    fmm::merge_plans(parts, pl->tree, pl->lists, with_groups ? &pl->groups : nullptr, fmmdev::kM2LChunkPairs);
+    pl->groups_kind = with_groups ? kind : -1;
+  } catch (const std::bad_alloc &) {
+    delete pl;
+    return FMM_ERR_MEMORY;
+  }
+  pl->n = (int)n;
+  for (int d = 0; d < 3; d++) {
+    pl->xmin[d] = plans[0]->xmin[d];
+    pl->xmax[d] = plans[0]->xmax[d];
+  }
+  for (int i = 0; i < k; i++) {
+    pl->t_bounds += plans[i]->t_bounds;
+    pl->t_tree += plans[i]->t_tree;
+    pl->t_lists += plans[i]->t_lists;
+  }
+  *out = pl;
+  return FMM_OK;
+}
 
 int fmm_plan_stats(const fmm_plan *pl, fmm_stats *s) {
   if (!pl || !s) return FMM_ERR_ARG;
@@ -413,15 +473,23 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
   UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
-  c->use_mfma = c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_mfma_supported(c->p.P) && L.nm2l > 0;
+  c->mixed = c->p.precision == FMM_MIXED;
+  c->use_mfma = c->p.m2l_mode == FMM_M2L_AUTO && L.nm2l > 0 &&
+                (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
   c->groups = fmm::M2LGroups();
   c->nchunk = 0;
   if (c->use_mfma) {
-    c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs);
+    if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64))
+      c->groups = std::move(pl.groups);
+    else
+      c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed);
     const fmm::M2LGroups &G = c->groups;
     c->nchunk = (int)(G.chunk.size() / 4);
     UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_dx, G.dx) UP(g_chunk, G.chunk)
     UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
+    if (c->mixed) {
+      UP(g_sc, G.scale) UP(cr, t.R)
+    }
   }
 #undef UP
   HIPCHK(hipStreamSynchronize(c->st));  // host staging vectors die here
@@ -431,6 +499,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->az, sizeof(double) * n))) return rc;
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
+  if (c->mixed && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
   double t4 = now_ms();
   pl.tree = fmm::Tree();
   plan_stats(&pl, c->stats);
@@ -561,9 +630,9 @@ int fmm_p2p(fmm_ctx *c) {
   }
   const double3 xper = make_double3(0.0, 0.0, 0.0);
   if (c->timing) HIPCHK(hipEventRecord(c->kev[2], c->st));
-  HIPCHK(fmmdev::launch_p2p(dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(),
-                            (int)c->lists.p2p_targets.size(), c->p2p_ptr.as<int>(),
-                            c->p2p_src.as<int>(), xper, c->p.eps2, c->st));
+  HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
+      dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
+      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->st));
   if (c->timing) HIPCHK(hipEventRecord(c->kev[3], c->st));
   return FMM_OK;
 }
@@ -576,9 +645,18 @@ int fmm_m2l(fmm_ctx *c) {
   const double3 xper = make_double3(0.0, 0.0, 0.0);
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   if (c->use_mfma) {
-    HIPCHK(fmmdev::launch_m2l_mfma(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
-                                   c->g_tgt.as<int>(), c->g_src.as<int>(), c->M.as<double2>(),
-                                   c->L.as<double2>(), c->st));
+    if (c->mixed) {
+      HIPCHK(fmmdev::launch_m_to_f32(c->p.P, c->M.as<double2>(), c->cr.as<double>(), c->tree.ncell,
+                                     c->M32.as<float2>(), c->st));
+      HIPCHK(fmmdev::launch_m2l_mfma32(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
+                                       c->g_sc.as<double2>(), c->g_tgt.as<int>(), c->g_src.as<int>(),
+                                       c->M32.as<float2>(), c->L.as<double2>(), c->st));
+    } else {
+      HIPCHK(fmmdev::launch_m2l_mfma(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
+                                     c->g_tgt.as<int>(), c->g_src.as<int>(), c->M.as<double2>(),
+                                     c->L.as<double2>(), c->st));
+    }
+    // pairs outside the offset keys (or, mixed, on levels of non-uniform radius): FP64 per target
     if (c->groups.nrest > 0)
       HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells(c), c->r_tgt.as<int>(),
                                 (int)c->groups.rest_targets.size(), c->r_ptr.as<int>(),
@@ -694,15 +772,16 @@ int fmm_get_targets(fmm_ctx *c, double *trg, int input_order) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!trg && c->n > 0) FMM_FAIL(FMM_ERR_ARG, "fmm_get_targets: NULL trg");
-  HIPCHK(hipStreamSynchronize(c->st));
   const int n = c->n;
   std::vector<double> h(4 * (size_t)n);
-  if (n > 0) {
-    HIPCHK(hipMemcpy(h.data(), c->pot.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h.data() + n, c->ax.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h.data() + 2 * (size_t)n, c->ay.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h.data() + 3 * (size_t)n, c->az.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (n > 0) {  // on the context's stream: other contexts' work is not waited for
+    const size_t b = sizeof(double) * n;
+    HIPCHK(hipMemcpyAsync(h.data(), c->pot.p, b, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(h.data() + n, c->ax.p, b, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(h.data() + 2 * (size_t)n, c->ay.p, b, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(h.data() + 3 * (size_t)n, c->az.p, b, hipMemcpyDeviceToHost, c->st));
   }
+  HIPCHK(hipStreamSynchronize(c->st));
   for (int i = 0; i < n; i++) {
     const size_t o = input_order ? (size_t)c->tree.perm[i] : (size_t)i;
     for (int k = 0; k < 4; k++) trg[4 * o + k] = h[(size_t)k * n + i];

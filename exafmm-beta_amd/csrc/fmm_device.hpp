@@ -52,6 +52,14 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
                       const double2 *L, hipStream_t s);
 // P2P for every target leaf over its CSR list (laplace.h:150-235); overwrites
 // the targets of each listed leaf
+// mixed precision (FP32 list phases): M32 = M~ / R^n per cell; grouped FP32-MFMA
+// M2L with per-group scales (R_s, R_t) accumulating into the FP64 L; FP32 P2P
+bool m2l_mixed_supported(int P);
+hipError_t launch_m_to_f32(int P, const double2 *M, const double *R, int ncell, float2 *M32, hipStream_t s);
+hipError_t launch_m2l_mfma32(int P, const int4 *chunks, int nchunk, const double *gdx, const double2 *gscale,
+                             const int *ptgt, const int *psrc, const float2 *M32, double2 *L, hipStream_t s);
+hipError_t launch_p2p32(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
+                        const int *src, double3 xper, double eps2, hipStream_t s);
 hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
                       const int *src, double3 xper, double eps2, hipStream_t s);
 

@@ -12,6 +12,8 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 namespace fmm {
 
@@ -253,6 +255,10 @@ struct Walker {
   int nspawn;
   double xper[3];
   std::vector<int> m2l_i, m2l_j, p2p_i, p2p_j;
+  // frontier mode: cell pairs holding at most `cut` bodies are recorded as
+  // tasks (in visiting order) instead of being traversed
+  int cut = 0;
+  std::vector<std::pair<int, int>> *tasks = nullptr;
 
   void dual(int Ci, int Cj) {
     // dualTreeTraversal (include/traversal.h:117-177)
@@ -260,7 +266,12 @@ struct Walker {
     for (int d = 0; d < 3; d++) dX[d] = t.X[3 * (size_t)Ci + d] - t.X[3 * (size_t)Cj + d] - xper[d];
     double RT2 = (dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2]) * theta * theta;
     double Rs = t.R[Ci] + t.R[Cj];
-    if (RT2 > Rs * Rs * (1 - 1e-3)) {
+    const bool accept = RT2 > Rs * Rs * (1 - 1e-3);
+    if (tasks && (accept || (t.nchild[Ci] == 0 && t.nchild[Cj] == 0) || t.nbody[Ci] + t.nbody[Cj] <= cut)) {
+      tasks->emplace_back(Ci, Cj);  // emitted pair or subtree pair: replayed in order by a worker
+      return;
+    }
+    if (accept) {
       m2l_i.push_back(Ci);
       m2l_j.push_back(Cj);
     } else if (t.nchild[Ci] == 0 && t.nchild[Cj] == 0) {
@@ -310,16 +321,56 @@ struct Walker {
   }
 };
 
-// Stable counting sort of (target, source) pairs by target: each target's
-// list keeps the traversal's visiting order.
-void to_csr(int ncell, const std::vector<int> &ti, const std::vector<int> &sj,
-            std::vector<int> &ptr, std::vector<int> &src, std::vector<int> &targets) {
+// Stable counting sort of (target, source) pairs by target over segments
+// taken in order: each target's list keeps the traversal's visiting order.
+struct Seg {
+  const std::vector<int> *ti, *sj;
+};
+void to_csr(int ncell, const std::vector<Seg> &segs, std::vector<int> &ptr, std::vector<int> &src,
+            std::vector<int> &targets, int nth) {
   ptr.assign(ncell + 1, 0);
-  for (int c : ti) ptr[c + 1]++;
-  for (int c = 0; c < ncell; c++) ptr[c + 1] += ptr[c];
-  src.resize(ti.size());
-  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-  for (size_t k = 0; k < ti.size(); k++) src[fill[ti[k]]++] = sj[k];
+  size_t total = 0;
+  for (const Seg &g : segs) total += g.ti->size();
+  src.resize(total);
+  // nth contiguous runs of segments, each counted and scattered by one thread
+  // into its own slots of every target's range (run order = segment order)
+  const int T = (int)std::min<size_t>(std::max(nth, 1), std::max<size_t>(segs.size(), 1));
+  std::vector<std::vector<int>> cnt(T, std::vector<int>(ncell, 0));
+  auto run_of = [&](int r, size_t &lo, size_t &hi) {
+    lo = segs.size() * r / T;
+    hi = segs.size() * (r + 1) / T;
+  };
+  auto par = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (int r = 1; r < T; r++) th.emplace_back(fn, r);
+    fn(0);
+    for (auto &x : th) x.join();
+  };
+  par([&](int r) {
+    size_t lo, hi;
+    run_of(r, lo, hi);
+    for (size_t g = lo; g < hi; g++)
+      for (int c : *segs[g].ti) cnt[r][c]++;
+  });
+  int64_t acc = 0;
+  for (int c = 0; c < ncell; c++) {
+    ptr[c] = (int)acc;
+    for (int r = 0; r < T; r++) {
+      const int k = cnt[r][c];
+      cnt[r][c] = (int)acc;  // becomes run r's first slot for target c
+      acc += k;
+    }
+  }
+  ptr[ncell] = (int)acc;
+  par([&](int r) {
+    size_t lo, hi;
+    run_of(r, lo, hi);
+    std::vector<int> &fill = cnt[r];
+    for (size_t g = lo; g < hi; g++) {
+      const std::vector<int> &ti = *segs[g].ti, &sj = *segs[g].sj;
+      for (size_t k = 0; k < ti.size(); k++) src[fill[ti[k]]++] = sj[k];
+    }
+  });
   targets.clear();
   for (int c = 0; c < ncell; c++)
     if (ptr[c + 1] > ptr[c]) targets.push_back(c);
@@ -327,26 +378,70 @@ void to_csr(int ncell, const std::vector<int> &ti, const std::vector<int> &sj,
 
 }  // namespace
 
+namespace {
+thread_local int tl_host_threads = 0;
+}
+
+void set_host_threads(int n) { tl_host_threads = n; }
+
+int host_threads() {
+  if (tl_host_threads > 0) return tl_host_threads;
+  const int hw = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(16, hw));
+}
+
 Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]) {
   Lists L;
   L.m2l_ptr.assign(t.ncell + 1, 0);
   L.p2p_ptr.assign(t.ncell + 1, 0);
   if (t.ncell == 0) return L;
-  Walker w{t, theta, nspawn, {0, 0, 0}, {}, {}, {}, {}};
-  if (xper) std::memcpy(w.xper, xper, sizeof w.xper);
-  w.m2l_i.reserve(64 * (size_t)t.ncell);
-  w.m2l_j.reserve(64 * (size_t)t.ncell);
-  w.p2p_i.reserve(32 * (size_t)t.leaves.size());
-  w.p2p_j.reserve(32 * (size_t)t.leaves.size());
-  w.dual(0, 0);
-  L.nm2l = (int64_t)w.m2l_i.size();
-  L.np2p = (int64_t)w.p2p_i.size();
-  to_csr(t.ncell, w.m2l_i, w.m2l_j, L.m2l_ptr, L.m2l_src, L.m2l_targets);
-  to_csr(t.ncell, w.p2p_i, w.p2p_j, L.p2p_ptr, L.p2p_src, L.p2p_targets);
-  int64_t bp = 0;
-  for (size_t k = 0; k < w.p2p_i.size(); k++)
-    bp += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
-  L.p2p_body_pairs = bp;
+  double xp[3] = {0, 0, 0};
+  if (xper) std::memcpy(xp, xper, sizeof xp);
+  const int nth = t.n >= 50000 ? host_threads() : 1;
+  std::vector<Walker> ws;
+  if (nth == 1) {
+    ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
+    Walker &w = ws[0];
+    w.m2l_i.reserve(64 * (size_t)t.ncell);
+    w.m2l_j.reserve(64 * (size_t)t.ncell);
+    w.p2p_i.reserve(32 * (size_t)t.leaves.size());
+    w.p2p_j.reserve(32 * (size_t)t.leaves.size());
+    w.dual(0, 0);
+  } else {
+    // The recursion down to cell pairs of at most `cut` bodies runs here and
+    // leaves the pairs below it as tasks in visiting order; threads traverse
+    // blocks of consecutive tasks, and the blocks' pairs are taken in block
+    // order, which is the serial visiting order.
+    std::vector<std::pair<int, int>> tasks;
+    Walker top{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}};
+    top.cut = std::max(2 * 64, t.n / (64 * nth));
+    top.tasks = &tasks;
+    top.dual(0, 0);
+    const size_t nblk = std::min<size_t>(tasks.size(), (size_t)nth * 16);
+    for (size_t b = 0; b < nblk; b++) ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      for (size_t b; (b = next.fetch_add(1)) < nblk;) {
+        const size_t lo = tasks.size() * b / nblk, hi = tasks.size() * (b + 1) / nblk;
+        for (size_t k = lo; k < hi; k++) ws[b].dual(tasks[k].first, tasks[k].second);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < nth; i++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+  }
+  std::vector<Seg> sm, sp;
+  for (const Walker &w : ws) {
+    L.nm2l += (int64_t)w.m2l_i.size();
+    L.np2p += (int64_t)w.p2p_i.size();
+    sm.push_back({&w.m2l_i, &w.m2l_j});
+    sp.push_back({&w.p2p_i, &w.p2p_j});
+    for (size_t k = 0; k < w.p2p_i.size(); k++)
+      L.p2p_body_pairs += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
+  }
+  to_csr(t.ncell, sm, L.m2l_ptr, L.m2l_src, L.m2l_targets, nth);
+  to_csr(t.ncell, sp, L.p2p_ptr, L.p2p_src, L.p2p_targets, nth);
   return L;
 }
 
@@ -404,7 +499,7 @@ struct KeyIndex {
 
 }  // namespace
 
-M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level) {
   M2LGroups G;
   const int nc = t.ncell;
   G.rest_ptr.assign(nc + 1, 0);
@@ -457,13 +552,45 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
       gid[p] = g;
     }
   }
-  const int ng = index.count;
-  G.ngroup = ng;
-  G.dx.resize(3 * (size_t)ng);
+  int ng = index.count;
+  std::vector<double> odx(3 * (size_t)ng);
   for (int g = 0; g < ng; g++) {
     const double u = R0 / std::ldexp(1.0, gexp[g]);
-    for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
+    for (int d = 0; d < 3; d++) odx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
   }
+  if (by_level) {
+    // split each offset group by (target level, source level)
+    KeyIndex lindex;
+    std::vector<int> base;  // offset group of each split group
+    std::vector<double> sc;
+    for (int c = 0; c < nc; c++) {
+      for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+        if (gid[p] < 0) continue;
+        const int sc_cell = L.m2l_src[p];
+        const int lt = t.level[c], ls = t.level[sc_cell];
+        const uint64_t key = ((uint64_t)gid[p] << 12) | ((uint64_t)lt << 6) | (uint64_t)ls;
+        bool fresh;
+        const int g2 = lindex.get(key, fresh);
+        if (fresh) {
+          base.push_back(gid[p]);
+          sc.insert(sc.end(), {t.R[sc_cell], t.R[c]});
+        }
+        if (t.R[sc_cell] != sc[2 * (size_t)g2] || t.R[c] != sc[2 * (size_t)g2 + 1]) {
+          gid[p] = -1;  // radius not uniform on its level: FP64 per-target path
+          continue;
+        }
+        gid[p] = g2;
+      }
+    }
+    ng = lindex.count;
+    G.dx.resize(3 * (size_t)ng);
+    for (int g = 0; g < ng; g++)
+      for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = odx[3 * (size_t)base[g] + d];
+    G.scale = std::move(sc);
+  } else {
+    G.dx = std::move(odx);
+  }
+  G.ngroup = ng;
   // counting sort by group, CSR order kept inside a group
   G.grp_ptr.assign(ng + 1, 0);
   for (int64_t p = 0; p < np; p++)
@@ -492,6 +619,12 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
   G.nrest = (int64_t)G.rest_src.size();
   for (int c = 0; c < nc; c++)
     if (G.rest_ptr[c + 1] > G.rest_ptr[c]) G.rest_targets.push_back(c);
+  deal_m2l_chunks(G, chunk_pairs);
+  return G;
+}
+
+void deal_m2l_chunks(M2LGroups &G, int chunk_pairs) {
+  const int ng = G.ngroup;
   // Chunks sorted by their first target (cell index ranges are subtrees, so
   // spatially compact), then dealt to the XCDs in contiguous runs: block b of
   // the launch runs on XCD b % kXcd, so XCD x gets the x-th eighth of the
@@ -513,7 +646,85 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs) {
       if (src < nch)
         for (int k = 0; k < 4; k++) G.chunk[4 * dst + k] = ch[src][k];
     }
-  return G;
+}
+
+void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs) {
+  T = Tree();
+  L = Lists();
+  int64_t nb = 0, nc = 0, npm = 0, npp = 0;
+  for (const PlanPart &p : parts) {
+    nb += p.t->n;
+    nc += p.t->ncell;
+    npm += p.l->nm2l;
+    npp += p.l->np2p;
+  }
+  T.n = (int)nb;
+  T.ncell = (int)nc;
+  L.nm2l = npm;
+  L.np2p = npp;
+  L.m2l_ptr.assign(nc + 1, 0);
+  L.p2p_ptr.assign(nc + 1, 0);
+  int bo = 0, co = 0, pmo = 0, ppo = 0;
+  for (const PlanPart &p : parts) {
+    const Tree &t = *p.t;
+    const Lists &l = *p.l;
+    T.depth = std::max(T.depth, t.depth);
+    for (int v : t.perm) T.perm.push_back(v + bo);
+    for (int c = 0; c < t.ncell; c++) {
+      T.iparent.push_back(t.iparent[c] + co);
+      T.ichild.push_back(t.ichild[c] + co);
+      T.nchild.push_back(t.nchild[c]);
+      T.ibody.push_back(t.ibody[c] + bo);
+      T.nbody.push_back(t.nbody[c]);
+      T.level.push_back(t.level[c]);
+      T.R.push_back(t.R[c]);
+    }
+    T.X.insert(T.X.end(), t.X.begin(), t.X.end());
+    for (int v : t.leaves) T.leaves.push_back(v + co);
+    for (int c = 0; c < t.ncell; c++) {
+      L.m2l_ptr[co + c + 1] = l.m2l_ptr[c + 1] + pmo;
+      L.p2p_ptr[co + c + 1] = l.p2p_ptr[c + 1] + ppo;
+    }
+    for (int v : l.m2l_src) L.m2l_src.push_back(v + co);
+    for (int v : l.p2p_src) L.p2p_src.push_back(v + co);
+    for (int v : l.m2l_targets) L.m2l_targets.push_back(v + co);
+    for (int v : l.p2p_targets) L.p2p_targets.push_back(v + co);
+    L.p2p_body_pairs += l.p2p_body_pairs;
+    bo += t.n;
+    co += t.ncell;
+    pmo += (int)l.nm2l;
+    ppo += (int)l.np2p;
+  }
+  // level index over the whole forest, cell order kept inside a level
+  T.level_ptr.assign(T.depth + 2, 0);
+  for (int c = 0; c < T.ncell; c++) T.level_ptr[T.level[c] + 1]++;
+  for (int l = 0; l <= T.depth; l++) T.level_ptr[l + 1] += T.level_ptr[l];
+  T.level_cells.resize(T.ncell);
+  std::vector<int> fill(T.level_ptr.begin(), T.level_ptr.end() - 1);
+  for (int c = 0; c < T.ncell; c++) T.level_cells[fill[T.level[c]]++] = c;
+  if (!G) return;
+  *G = M2LGroups();
+  G->grp_ptr.assign(1, 0);
+  G->rest_ptr.assign(nc + 1, 0);
+  co = 0;
+  int po = 0;
+  for (const PlanPart &p : parts) {
+    const M2LGroups &g = *p.g;
+    for (int k = 0; k < g.ngroup; k++) G->grp_ptr.push_back(g.grp_ptr[k + 1] + po);
+    for (int v : g.tgt) G->tgt.push_back(v + co);
+    for (int v : g.src) G->src.push_back(v + co);
+    G->dx.insert(G->dx.end(), g.dx.begin(), g.dx.end());
+    G->scale.insert(G->scale.end(), g.scale.begin(), g.scale.end());
+    const int ro = (int)G->rest_src.size();
+    for (int c = 0; c < p.t->ncell; c++) G->rest_ptr[co + c + 1] = g.rest_ptr[c + 1] + ro;
+    for (int v : g.rest_src) G->rest_src.push_back(v + co);
+    for (int v : g.rest_targets) G->rest_targets.push_back(v + co);
+    G->ngroup += g.ngroup;
+    co += p.t->ncell;
+    po += (int)g.tgt.size();
+  }
+  G->nrest = (int64_t)G->rest_src.size();
+  deal_m2l_chunks(*G, chunk_pairs);
 }
 
 namespace {

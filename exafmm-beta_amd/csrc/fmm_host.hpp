@@ -36,6 +36,12 @@ struct Lists {
   int64_t p2p_body_pairs = 0;          // sum over P2P pairs of Ni*Nj
 };
 
+// Host threads for one tree/list/group build: n > 0 fixes the count for the
+// calling thread (1 = serial, as the request loop's plan builders use);
+// 0 = hardware threads, at most 16.
+void set_host_threads(int n);
+int host_threads();
+
 // BoundBox::getBounds (include/bound_box.h:11-25): sequential scan that widens
 // the box by 1e-5 per body, as the reference does.
 void get_bounds(int n, const double *x, double xmin[3], double xmax[3]);
@@ -70,8 +76,26 @@ struct M2LGroups {
   std::vector<int> chunk;          // 4 per chunk
   std::vector<int> rest_ptr, rest_src, rest_targets;
   int64_t nrest = 0;
+  std::vector<double> scale;       // 2 per group (R_s, R_t) when grouped by level
 };
-M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs);
+// by_level: a group also fixes the source and target levels (the scaled FP32
+// operator depends on R_s and R_t); pairs whose radii differ from their
+// group's are left ungrouped.
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level = false);
+// (Re)build G.chunk from the groups: chunks of at most chunk_pairs pairs,
+// ordered by first target and dealt round-robin-contiguously to the 8 XCDs.
+void deal_m2l_chunks(M2LGroups &G, int chunk_pairs);
+
+// Several independent systems as one forest (batched evaluation): cell, body
+// and pair indices of part k are offset by the sizes of parts 0..k-1, levels
+// are kept, and no pair joins two systems.  G (optional) merges the parts'
+// M2L groups (every part must carry groups of the same kind).
+struct PlanPart {
+  const Tree *t;
+  const Lists *l;
+  const M2LGroups *g;
+};
+void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs);
 
 // Dataset generators (include/dataset.h:52-171, 220-227): drand48 streams
 // seeded with srand48(seed), positions first, then charges after reseeding,

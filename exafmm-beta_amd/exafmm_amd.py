@@ -25,12 +25,40 @@ SYMBOLS = [
     "fmm_synchronize", "fmm_get_targets", "fmm_get_expansions", "fmm_get_stats", "fmm_get_tree",
     "fmm_get_lists", "fmm_set_bodies_plan", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
     "fmm_plan_tree", "fmm_plan_lists", "fmm_make_bodies",
-    "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree",
+    "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
+    "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
+    "fmm_server_get_stats", "fmm_server_destroy",
 ]
 
 FMM_FP64 = 0
+FMM_MIXED = 1
 M2L_AUTO = 0      # offset-grouped FP64 MFMA where supported (P <= 10)
 M2L_TARGET = 1    # per-target kernel, fixed summation order
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("host_threads", ctypes.c_int), ("slots", ctypes.c_int),
+                ("device_budget", ctypes.c_int64), ("batch_bodies", ctypes.c_int),
+                ("batch_requests", ctypes.c_int), ("batch_delay_ms", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("batch_requests", ctypes.c_int), ("batch_bodies", ctypes.c_int),
+                ("slot", ctypes.c_int), ("t_submit_ms", ctypes.c_double),
+                ("t_planned_ms", ctypes.c_double), ("t_started_ms", ctypes.c_double),
+                ("t_done_ms", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests_submitted", ctypes.c_int64), ("requests_done", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("bodies_done", ctypes.c_int64),
+                ("max_batch_requests", ctypes.c_int), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FMMError(RuntimeError):
@@ -105,6 +133,17 @@ def load():
         L.fmm_plan_from_tree.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
                                          _ip, _ip, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_char_p, ctypes.c_int]
+        L.fmm_plan_build_groups.argtypes = [vp, ctypes.c_int]
+        L.fmm_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.fmm_server_default_config.restype = None
+        L.fmm_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params),
+                                        ctypes.POINTER(ServerConfig)]
+        L.fmm_server_submit.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]
+        L.fmm_server_wait.argtypes = [vp, ctypes.c_int64, _dp, ctypes.POINTER(RequestInfo)]
+        L.fmm_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.fmm_server_destroy.argtypes = [vp]
+        L.fmm_server_destroy.restype = None
+        L.fmm_plan_merge.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
         _lib = L
     return _lib
 
@@ -303,6 +342,25 @@ class Plan:
         except Exception:
             pass
 
+    @classmethod
+    def merge(cls, plans):
+        """Forest of independent systems (fmm_plan_merge); bodies are the
+        plans' x concatenated in order."""
+        L = load()
+        self = cls.__new__(cls)
+        self.x = np.concatenate([p.x for p in plans]) if plans else np.zeros((0, 3))
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(plans))(*[p._h.value for p in plans])
+        rc = L.fmm_plan_merge(ctypes.byref(self._h), len(plans), arr)
+        if rc != 0:
+            raise FMMError("fmm_plan_merge failed (rc=%d)" % rc)
+        return self
+
+    def build_groups(self, precision=FMM_FP64):
+        rc = load().fmm_plan_build_groups(self._h, precision)
+        if rc != 0:
+            raise FMMError("fmm_plan_build_groups failed (rc=%d)" % rc)
+
     def stats(self):
         s = Stats()
         load().fmm_plan_stats(self._h, ctypes.byref(s))
@@ -331,3 +389,62 @@ class Plan:
         load().fmm_plan_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip), _ptr(p2p_ptr, _ip),
                               _ptr(p2p_src, _ip))
         return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]
+
+
+class Server:
+    """Request loop (fmm_server_*): submit systems as they arrive, wait for
+    each result.  Keyword arguments of FMM give the solve parameters;
+    host_threads, slots, device_budget, batch_bodies, batch_requests the loop."""
+
+    def __init__(self, P=8, theta=0.4, ncrit=64, nspawn=5000, eps2=0.0, device=0,
+                 precision=FMM_FP64, m2l_mode=M2L_AUTO, **cfg):
+        L = load()
+        p = Params()
+        L.fmm_default_params(ctypes.byref(p))
+        p.P, p.theta, p.ncrit, p.nspawn, p.eps2 = P, theta, ncrit, nspawn, eps2
+        p.device, p.precision, p.m2l_mode = device, precision, m2l_mode
+        c = ServerConfig()
+        L.fmm_server_default_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self._h = ctypes.c_void_p()
+        self._n = {}
+        rc = L.fmm_server_create(ctypes.byref(self._h), ctypes.byref(p), ctypes.byref(c))
+        if rc != 0:
+            self._h = ctypes.c_void_p()
+            raise FMMError("fmm_server_create failed (rc=%d)" % rc)
+
+    def submit(self, x, q):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        rid = ctypes.c_int64()
+        rc = load().fmm_server_submit(self._h, len(x), _ptr(x), _ptr(q), ctypes.byref(rid))
+        if rc != 0:
+            raise FMMError("fmm_server_submit failed (rc=%d)" % rc)
+        self._n[rid.value] = len(x)
+        return rid.value
+
+    def wait(self, rid):
+        """(targets (n,4) in input order, request info dict)"""
+        trg = np.zeros((self._n.pop(rid), 4))
+        info = RequestInfo()
+        rc = load().fmm_server_wait(self._h, rid, _ptr(trg), ctypes.byref(info))
+        if rc != 0:
+            raise FMMError("fmm_server_wait failed (rc=%d)" % rc)
+        return trg, info.as_dict()
+
+    def stats(self):
+        s = ServerStats()
+        load().fmm_server_get_stats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def close(self):
+        if self._h:
+            load().fmm_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -31,7 +31,13 @@ typedef enum {
   FMM_ERR_UNSUPPORTED = -5
 } fmm_status;
 
-typedef enum { FMM_FP64 = 0 } fmm_precision;
+/* FP64: every phase in FP64 (parity with the reference to 1e-10).
+ * MIXED: P2P and M2L in FP32 (P2P with FP32 partial sums of at most 32 terms
+ * added into FP64 accumulators; M2L as grouped FP32 MFMA translations whose
+ * results are added into FP64 local expansions); P2M, M2M, L2L, L2P, the
+ * harmonics of every translation and all accumulation across lists stay FP64.
+ * Needs P <= 12 and m2l_mode AUTO. */
+typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
 
 /* M2L execution: AUTO uses the offset-grouped FP64 MFMA kernel where the order
  * allows (P <= 10) and the per-target kernel otherwise; TARGET always uses the
@@ -131,6 +137,19 @@ int fmm_plan_from_tree(fmm_plan **out, int n, const double *x, int ncell, const 
                        const double *X, const double *R, double theta, int nspawn, char *why,
                        int why_len);
 
+/* Precompute the plan's M2L offset groups for a context of this precision
+ * (FMM_FP64: by offset; FMM_MIXED: by offset and the two levels), so that a
+ * context given the plan does no grouping work itself.  Host only. */
+int fmm_plan_build_groups(fmm_plan *pl, int precision);
+
+/* Batched evaluation of independent systems: one plan for the forest of the
+ * k plans (cells, bodies and pairs of plan i follow those of plans 0..i-1; no
+ * interaction crosses systems).  Bodies of the merged plan are the k input
+ * arrays concatenated in order.  Prebuilt groups are merged when every plan
+ * has the same kind.  No counterpart in the reference (one system per solve
+ * there); host only. */
+int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans);
+
 /* Re-upload charges only (same positions, same tree and lists). */
 int fmm_set_charges(fmm_ctx *c, const double *q);
 
@@ -164,6 +183,54 @@ int fmm_get_stats(fmm_ctx *c, fmm_stats *s);
 int fmm_get_tree(fmm_ctx *c, int *perm, int *iparent, int *ichild, int *nchild, int *ibody,
                  int *nbody, double *X, double *R);
 int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src);
+
+/* ------------------------------------------------------------------ serving
+ * A request loop for streams of independent systems (successive MD steps,
+ * many small problems).  No counterpart in the reference, which solves one
+ * system per call.  Submitted systems are planned on host threads (bounds,
+ * tree, lists, M2L groups) and run in batches: a free slot (one context and
+ * HIP stream) takes the oldest planned requests -- up to batch_bodies bodies
+ * or batch_requests requests -- once their device footprint fits
+ * device_budget beside the batches in flight, and evaluates them as one
+ * forest (one launch per phase for the whole batch).  Batches of different
+ * slots overlap on the GPU.  A system's result does not depend on its batch
+ * beyond the order of FP64 additions into its local expansions. */
+typedef struct fmm_server fmm_server;
+
+typedef struct {
+  int host_threads;       /* plan builders; 0: hardware threads - slots (max 16) */
+  int slots;              /* batches in flight (contexts/streams), 1..16 */
+  int64_t device_budget;  /* bytes the in-flight batches may occupy */
+  int batch_bodies;       /* a batch closes at this many bodies ... */
+  int batch_requests;     /* ... or this many requests */
+  double batch_delay_ms;  /* hold a batch open up to this long for more requests (0: never) */
+} fmm_server_config;
+
+typedef struct {
+  int n;
+  int batch_requests, batch_bodies, slot;  /* the batch the request ran in */
+  double t_submit_ms, t_planned_ms, t_started_ms, t_done_ms;  /* server clock */
+  double device_ms;                        /* device time of that batch */
+} fmm_request_info;
+
+typedef struct {
+  int64_t requests_submitted, requests_done, batches, bodies_done;
+  int max_batch_requests;
+  double device_ms;
+} fmm_server_stats;
+
+void fmm_server_default_config(fmm_server_config *cfg);
+/* p as for fmm_create (validated once here); cfg NULL: defaults */
+int fmm_server_create(fmm_server **out, const fmm_params *p, const fmm_server_config *cfg);
+/* copies x (n,3) and q; returns at once with the request id */
+int fmm_server_submit(fmm_server *s, int n, const double *x, const double *q, int64_t *id);
+/* blocks until request id is done; trg (n,4) in the request's input order;
+ * info optional.  Each id is waited for once. */
+int fmm_server_wait(fmm_server *s, int64_t id, double *trg, fmm_request_info *info);
+int fmm_server_get_stats(fmm_server *s, fmm_server_stats *out);
+/* stops the threads after the batches in flight; call after the last wait
+ * (requests still queued are dropped) */
+void fmm_server_destroy(fmm_server *s);
 
 #ifdef __cplusplus
 }

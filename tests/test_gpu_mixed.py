@@ -1,0 +1,63 @@
+"""Mixed precision (FMM_MIXED: FP32 P2P and FP32-MFMA M2L, FP64 elsewhere)
+against the FP64 path and the FP64 direct sum.
+
+Tolerances: the FP32 list phases must stay well below the FMM truncation
+error they sit beside -- rel-L2 distance to the FP64 FMM below 1e-6 for the
+potential and 5e-6 for the acceleration, and an error against the FP64 direct
+sum within 1e-6 (pot) / 5e-6 (acc) of the FP64 FMM's own error.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+POT_TOL, ACC_TOL = 1e-6, 5e-6
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+
+
+def _solve(F, x, q, P, precision, **kw):
+    f = F.FMM(P=P, precision=precision, **kw)
+    f.set_bodies(x, q)
+    f.evaluate()
+    out = f.targets(input_order=True), f.stats()
+    f.close()
+    return out
+
+
+@pytest.mark.parametrize("P,dist,n", [(4, "cube1", 20000), (8, "cube1", 40000), (10, "plummer1", 40000),
+                                      (12, "cube1", 40000)])
+def test_mixed_matches_fp64(P, dist, n):
+    F = _F()
+    x, q = F.make_bodies(n, dist, 1)
+    t64, s64 = _solve(F, x, q, P, F.FMM_FP64)
+    t32, s32 = _solve(F, x, q, P, F.FMM_MIXED)
+    assert s32["m2l_groups"] > 0 and s32["m2l_ungrouped"] == 0
+    dp, da = _rel(t32[:, 0], t64[:, 0]), _rel(t32[:, 1:], t64[:, 1:])
+    idx = np.arange(1000) * (n // 1000)
+    d = O.direct(O.Kernel(P), x[idx], x, q)
+    e64, e32 = O.rel_l2(d, t64[idx]), O.rel_l2(d, t32[idx])
+    print("P=%d %s n=%d: mixed-fp64 %.2e %.2e; vs direct fp64 %.2e %.2e mixed %.2e %.2e"
+          % (P, dist, n, dp, da, e64[0], e64[1], e32[0], e32[1]))
+    assert dp < POT_TOL and da < ACC_TOL
+    assert e32[0] < e64[0] + POT_TOL and e32[1] < e64[1] + ACC_TOL
+
+
+def test_mixed_single_leaf_and_empty():
+    F = _F()
+    x, q = F.make_bodies(50, "cube1", 2)
+    t32, s = _solve(F, x, q, 8, F.FMM_MIXED)
+    assert s["m2l_pairs"] == 0
+    d = O.direct(O.Kernel(8), x, x, q)
+    assert _rel(t32[:, 0], d[:, 0]) < POT_TOL and _rel(t32[:, 1:], d[:, 1:]) < ACC_TOL
+    t0, _ = _solve(F, np.zeros((0, 3)), np.zeros(0), 8, F.FMM_MIXED)
+    assert t0.shape == (0, 4)

@@ -1,0 +1,86 @@
+"""The request loop (fmm_server_*): every result is the same alone and in a
+batch, batching and slot overlap actually happen, and memory admission holds.
+
+Tolerance alone vs in the mix: 1e-12 relative (L2 per request).  A system's
+cells, lists and M2L chunks are identical in both cases; only the order of
+the FP64 atomic additions into its local expansions differs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIX_TOL = 1e-12
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+def _rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def _systems(F, k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        n = int(2 ** rng.uniform(9, 14))
+        x, q = F.make_bodies(n, "cube1" if i % 2 == 0 else "plummer1", 100 + i)
+        out.append((x, q))
+    return out
+
+
+def _serve(F, systems, precision, **cfg):
+    s = F.Server(P=8, precision=precision, **cfg)
+    ids = [s.submit(x, q) for x, q in systems]
+    res = [s.wait(i) for i in ids]
+    st = s.stats()
+    s.close()
+    return res, st
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_results_equal_alone_and_in_mix(precision):
+    F = _F()
+    prec = F.FMM_MIXED if precision == "mixed" else F.FMM_FP64
+    systems = _systems(F, 12, 1)
+    alone, st1 = _serve(F, systems, prec, slots=1, batch_requests=1)
+    mix, st2 = _serve(F, systems, prec, slots=3, batch_requests=64, batch_delay_ms=500.0)
+    assert st1["max_batch_requests"] == 1 and st1["batches"] == 12
+    assert st2["max_batch_requests"] > 1 and st2["requests_done"] == 12
+    for (ta, ia), (tm, im), (x, q) in zip(alone, mix, systems):
+        assert ta.shape == (len(x), 4)
+        assert _rel(tm[:, 0], ta[:, 0]) < MIX_TOL and _rel(tm[:, 1:], ta[:, 1:]) < MIX_TOL
+    # and equal to a context solving the system by itself
+    for (ta, _), (x, q) in zip(alone[:3], systems[:3]):
+        f = F.FMM(P=8, precision=prec)
+        f.set_bodies(x, q)
+        f.evaluate()
+        t = f.targets(input_order=True)
+        f.close()
+        assert _rel(ta[:, 0], t[:, 0]) < MIX_TOL and _rel(ta[:, 1:], t[:, 1:]) < MIX_TOL
+
+
+def test_memory_admission_serialises_batches():
+    """A budget below any footprint admits a batch only when none is in
+    flight: batch intervals never overlap."""
+    F = _F()
+    systems = _systems(F, 8, 2)
+    res, st = _serve(F, systems, F.FMM_FP64, slots=3, batch_requests=2, device_budget=1)
+    spans = sorted({(i["t_started_ms"], i["t_done_ms"]) for _, i in res})
+    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+        assert b0 >= a1
+    assert st["requests_done"] == 8
+
+
+def test_empty_and_tiny_requests():
+    F = _F()
+    s = F.Server(P=6, slots=2)
+    ids = [s.submit(np.zeros((0, 3)), np.zeros(0)), s.submit(np.array([[0.1, 0.2, 0.3]]), np.array([1.0]))]
+    t0, _ = s.wait(ids[0])
+    t1, _ = s.wait(ids[1])
+    s.close()
+    assert t0.shape == (0, 4) and np.array_equal(t1, np.zeros((1, 4)))

@@ -43,7 +43,9 @@ def _oracle_csr(t, ci, cj, ty, typ):
 @pytest.mark.parametrize("dist,n,ncrit,theta,nspawn", [
     ("cube", 32768, 64, 0.4, 5000), ("plummer", 20000, 64, 0.4, 5000),
     ("plummer", 20000, 8, 0.25, 100), ("cube", 5000, 16, 0.6, 1000), ("cube", 1, 64, 0.4, 5000),
-    ("cube", 64, 64, 0.4, 5000)])
+    ("cube", 64, 64, 0.4, 5000),
+    # above 50000 bodies the lists are built by several threads
+    ("plummer", 120000, 64, 0.4, 5000), ("cube", 150000, 32, 0.5, 1000)])
 def test_plan_matches_oracle(dist, n, ncrit, theta, nspawn):
     F = _F()
     x = O.data_cube(n, 2) if dist == "cube" else O.data_plummer(n, 2)
@@ -108,3 +110,50 @@ def test_create_reports_errors():
         F.FMM(P=1)            # unsupported order, rejected before any device call
     with pytest.raises(F.FMMError):
         F.FMM(P=10, theta=0)  # invalid parameter
+    with pytest.raises(F.FMMError):
+        F.FMM(P=14, precision=F.FMM_MIXED)  # FP32 M2L is built for P <= 12
+    with pytest.raises(F.FMMError):
+        F.FMM(P=8, precision=F.FMM_MIXED, m2l_mode=F.M2L_TARGET)  # mixed needs grouped M2L
+
+
+def test_plan_merge_is_a_forest():
+    F = _F()
+    xs = [O.data_cube(3000, 1), O.data_plummer(5000, 2), O.data_cube(40, 3)]
+    plans = [F.Plan(x, ncrit=32) for x in xs]
+    for p in plans:
+        p.build_groups(F.FMM_MIXED)
+    m = F.Plan.merge(plans)
+    tr, mp, ms, pp, ps = m.tree(), *m.lists()
+    bo = co = 0
+    for x, p in zip(xs, plans):
+        t = p.tree()
+        nc, n = len(t["R"]), len(x)
+        assert np.array_equal(tr["perm"][bo:bo + n], t["perm"] + bo)
+        for k in ("iparent", "ichild", "ibody"):
+            off = bo if k == "ibody" else co
+            assert np.array_equal(tr[k][co:co + nc], t[k] + off), k
+        assert np.array_equal(tr["nbody"][co:co + nc], t["nbody"])
+        assert np.array_equal(tr["X"][co:co + nc], t["X"]) and np.array_equal(tr["R"][co:co + nc], t["R"])
+        a, b, c, d = p.lists()
+        assert np.array_equal(np.diff(mp[co:co + nc + 1]), np.diff(a))
+        assert np.array_equal(ms[mp[co]:mp[co + nc]], b + co)
+        assert np.array_equal(ps[pp[co]:pp[co + nc]], d + co)
+        bo += n
+        co += nc
+    s = m.stats()
+    assert s["n"] == sum(len(x) for x in xs) and s["ncell"] == co
+    assert s["depth"] == max(p.stats()["depth"] for p in plans)
+    assert s["p2p_body_pairs"] == sum(p.stats()["p2p_body_pairs"] for p in plans)
+
+
+def test_server_without_device_reports_error():
+    """No GPU in this process: the server refuses to start with an error code
+    (its parameters are checked with a context first) instead of crashing."""
+    F = _F()
+    import os
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible here")
+    with pytest.raises(F.FMMError):
+        F.Server(P=8)
+    with pytest.raises(F.FMMError):
+        F.Server(P=1)
