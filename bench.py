@@ -168,11 +168,11 @@ def cpu_baseline(x, q, w, budget_s):
     }
 
 
-def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log):
+def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0):
     """Upload one system (host plan shared), run `warmup` + `steps` solves, and
     return timings, kernel figures and the accuracy on sampled targets."""
     n, P = w["n"], w["P"]
-    f = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"],
+    f = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], m2l_mode=m2l_mode,
               precision=F.FMM_MIXED if precision == "mixed" else F.FMM_FP64)
     f.set_bodies_plan(plan, q)
     for _ in range(warmup):
@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer1m-p10-fp64", choices=sorted(WORKLOADS) + sorted(SERVE))
     ap.add_argument("--requests", type=int, default=0, help="serve-*: number of requests (default 2048)")
+    ap.add_argument("--m2l-mode", type=int, default=0, choices=(0, 1, 2),
+                    help="M2L execution: 0 auto (target tiles), 1 per target, 2 grouped chunks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -335,7 +337,7 @@ def main():
     log("[bench] %s: ncell=%d depth=%d m2l=%d p2p=%d body_pairs=%d; host plan %.2f s"
         % (args.workload, st0["ncell"], st0["depth"], st0["m2l_pairs"], st0["p2p_pairs"],
            st0["p2p_body_pairs"], t_plan))
-    r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log)
+    r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log, args.m2l_mode)
     st = r["st"]
     kernels = r["kernels"]
     dom = "m2l" if kernels["m2l"]["avg_ms"] >= kernels["p2p"]["avg_ms"] else "p2p"
@@ -361,7 +363,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "P": P, "theta": w["theta"],
                    "ncrit": w["ncrit"], "nspawn": w["nspawn"], "inputs": w["desc"],
-                   "precision": w["precision"]},
+                   "precision": w["precision"], "m2l_mode": ("auto", "target", "grouped")[args.m2l_mode]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(r["lat"], 50)),
                        "p99": 1e3 * float(np.percentile(r["lat"], 99))},
         "device_ms": {"total": float(np.mean(r["dev"])), "upward": st["dev_upward_ms"],
@@ -381,7 +383,7 @@ def main():
     if w["precision"] == "mixed":
         # the FP64 solve of the same system, for "mixed vs FP64 at equal L2 error"
         steps64 = max(1, args.steps // 2)
-        r64 = solve_and_time(F, w, plan, x, q, "fp64", steps64, 1, log)
+        r64 = solve_and_time(F, w, plan, x, q, "fp64", steps64, 1, log, args.m2l_mode)
         v64 = n * steps64 / r64["wall_s"]
         out["fp64_same_system"] = {
             "value": v64, "ms_per_step": 1e3 * r64["wall_s"] / steps64,

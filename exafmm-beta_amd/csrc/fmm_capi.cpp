@@ -55,11 +55,12 @@ struct fmm_ctx {
   DBuf M, L;
   DBuf leaves, internal, level_cells, body_leaf;
   DBuf m2l_ptr, m2l_src, m2l_tgt, p2p_ptr, p2p_src, p2p_tgt;
-  // offset-grouped M2L
-  bool use_mfma = false;
-  int nchunk = 0;
+  // offset-grouped M2L: chunks (GROUPED) or target tiles (AUTO)
+  bool use_mfma = false, use_tiles = false;
+  int nchunk = 0, ntile = 0;
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
+  DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y;
   // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
   bool mixed = false;
   DBuf g_sc, cr, M32;
@@ -71,6 +72,7 @@ struct fmm_plan {
   fmm::Lists lists;
   fmm::M2LGroups groups;  // prebuilt M2L groups (fmm_plan_build_groups), kind:
   int groups_kind = -1;   // -1 none, FMM_FP64 offset groups, FMM_MIXED by level
+  fmm::M2LTiles tiles;    // prebuilt target tiles of those groups (TT > 0)
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   double t_bounds = 0, t_tree = 0, t_lists = 0;
 };
@@ -179,15 +181,15 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
   } else if (p->images != 0) {
     c->err = "periodic images are not supported (images must be 0)";
     rc = FMM_ERR_UNSUPPORTED;
-  } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET) {
+  } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET && p->m2l_mode != FMM_M2L_GROUPED) {
     c->err = "invalid m2l_mode";
     rc = FMM_ERR_ARG;
   } else if (p->precision != FMM_FP64 && p->precision != FMM_MIXED) {
     c->err = "unsupported precision mode";
     rc = FMM_ERR_UNSUPPORTED;
   } else if (p->precision == FMM_MIXED &&
-             (!fmmdev::m2l_mixed_supported(p->P) || p->m2l_mode != FMM_M2L_AUTO)) {
-    c->err = "mixed precision needs 2 <= P <= 12 and m2l_mode AUTO";
+             (!fmmdev::m2l_mixed_supported(p->P) || p->m2l_mode == FMM_M2L_TARGET)) {
+    c->err = "mixed precision needs 2 <= P <= 12 and m2l_mode AUTO or GROUPED";
     rc = FMM_ERR_UNSUPPORTED;
   }
   if (rc == FMM_OK) {
@@ -306,10 +308,31 @@ int fmm_plan_build_groups(fmm_plan *pl, int precision) {
   if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED)) return FMM_ERR_ARG;
   try {
     pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED);
+    pl->tiles = fmm::M2LTiles();
   } catch (const std::bad_alloc &) {
     return FMM_ERR_MEMORY;
   }
   pl->groups_kind = precision;
+  return FMM_OK;
+}
+
+int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
+  if (!pl || !p || (p->precision != FMM_FP64 && p->precision != FMM_MIXED)) return FMM_ERR_ARG;
+  if (p->m2l_mode == FMM_M2L_TARGET) return FMM_OK;
+  const bool mixed = p->precision == FMM_MIXED;
+  if (!(mixed ? fmmdev::m2l_mixed_supported(p->P) : fmmdev::m2l_mfma_supported(p->P))) return FMM_OK;
+  try {
+    if (pl->groups_kind != p->precision) {
+      pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, mixed);
+      pl->groups_kind = p->precision;
+      pl->tiles = fmm::M2LTiles();
+    }
+    const int TT = fmmdev::m2l_tile_targets(p->P);
+    if (p->m2l_mode == FMM_M2L_AUTO && pl->tiles.TT != TT)
+      pl->tiles = fmm::build_m2l_tiles(pl->tree, pl->groups, TT, 16);
+  } catch (const std::bad_alloc &) {
+    return FMM_ERR_MEMORY;
+  }
   return FMM_OK;
 }
 
@@ -322,7 +345,7 @@ int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans) {
   for (int i = 0; i < k; i++) {
     if (!plans[i]) return FMM_ERR_ARG;
     n += plans[i]->n;
-    parts.push_back({&plans[i]->tree, &plans[i]->lists, &plans[i]->groups});
+    parts.push_back({&plans[i]->tree, &plans[i]->lists, &plans[i]->groups, &plans[i]->tiles});
   }
   if (n > INT32_MAX) return FMM_ERR_ARG;
   bool same_kind = true;
@@ -331,7 +354,11 @@ int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans) {
   if (!pl) return FMM_ERR_MEMORY;
   try {
     const bool with_groups = same_kind && kind >= 0;
-    fmm::merge_plans(parts, pl->tree, pl->lists, with_groups ? &pl->groups : nullptr, fmmdev::kM2LChunkPairs);
+    bool with_tiles = with_groups;
+    for (int i = 0; i < k; i++) with_tiles = with_tiles && plans[i]->tiles.TT == plans[0]->tiles.TT;
+    with_tiles = with_tiles && plans[0]->tiles.TT > 0;
+    fmm::merge_plans(parts, pl->tree, pl->lists, with_groups ? &pl->groups : nullptr, fmmdev::kM2LChunkPairs,
+                     with_tiles ? &pl->tiles : nullptr);
     pl->groups_kind = with_groups ? kind : -1;
   } catch (const std::bad_alloc &) {
     delete pl;
@@ -474,19 +501,31 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
   UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
   c->mixed = c->p.precision == FMM_MIXED;
-  c->use_mfma = c->p.m2l_mode == FMM_M2L_AUTO && L.nm2l > 0 &&
+  c->use_mfma = c->p.m2l_mode != FMM_M2L_TARGET && L.nm2l > 0 &&
                 (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
+  c->use_tiles = c->use_mfma && c->p.m2l_mode == FMM_M2L_AUTO;
   c->groups = fmm::M2LGroups();
   c->nchunk = 0;
+  c->ntile = 0;
+  fmm::M2LTiles tiles;
   if (c->use_mfma) {
-    if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64))
+    if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64)) {
       c->groups = std::move(pl.groups);
-    else
+      if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) tiles = std::move(pl.tiles);
+    } else {
       c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed);
+    }
     const fmm::M2LGroups &G = c->groups;
-    c->nchunk = (int)(G.chunk.size() / 4);
-    UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_dx, G.dx) UP(g_chunk, G.chunk)
-    UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
+    UP(g_dx, G.dx) UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
+    if (c->use_tiles) {
+      if (tiles.TT == 0) tiles = fmm::build_m2l_tiles(t, G, fmmdev::m2l_tile_targets(c->p.P), 16);
+      c->ntile = (int)(tiles.item.size() / 4);  // work items
+      UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item) UP(t_slice, tiles.slice) UP(t_src, tiles.src)
+      UP(t_lt, tiles.lt)
+    } else {
+      c->nchunk = (int)(G.chunk.size() / 4);
+      UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_chunk, G.chunk)
+    }
     if (c->mixed) {
       UP(g_sc, G.scale) UP(cr, t.R)
     }
@@ -500,6 +539,12 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
   if (c->mixed && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
+  if (c->use_tiles) {  // harmonics of every group offset, once per plan
+    const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * c->groups.ngroup;
+    if ((rc = ensure(c, c->g_y, sizeof(double2) * ny))) return rc;
+    HIPCHK(fmmdev::launch_group_harmonics(c->p.P, c->g_dx.as<double>(), c->groups.ngroup, c->g_y.as<double2>(),
+                                          c->st));
+  }
   double t4 = now_ms();
   pl.tree = fmm::Tree();
   plan_stats(&pl, c->stats);
@@ -514,7 +559,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   s.host_upload_ms = t4 - t3;
   s.m2l_groups = c->groups.ngroup;
   s.m2l_ungrouped = c->use_mfma ? c->groups.nrest : L.nm2l;
-  s.m2l_path = c->use_mfma ? FMM_M2L_AUTO : FMM_M2L_TARGET;
+  s.m2l_path = c->use_tiles ? FMM_M2L_AUTO : c->use_mfma ? FMM_M2L_GROUPED : FMM_M2L_TARGET;
   c->have_bodies = true;
   return FMM_OK;
 }
@@ -645,9 +690,15 @@ int fmm_m2l(fmm_ctx *c) {
   const double3 xper = make_double3(0.0, 0.0, 0.0);
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   if (c->use_mfma) {
-    if (c->mixed) {
+    if (c->mixed)
       HIPCHK(fmmdev::launch_m_to_f32(c->p.P, c->M.as<double2>(), c->cr.as<double>(), c->tree.ncell,
                                      c->M32.as<float2>(), c->st));
+    if (c->use_tiles) {
+      HIPCHK(fmmdev::launch_m2l_tile(c->p.P, c->mixed, c->ntile, c->t_tgt.as<int>(), c->t_ptr.as<int4>(),
+                                     c->t_slice.as<int4>(), c->t_src.as<int>(), c->t_lt.as<unsigned char>(),
+                                     c->g_y.as<double2>(), c->g_sc.as<double2>(), c->cr.as<double>(),
+                                     c->mixed ? c->M32.p : c->M.p, c->L.as<double2>(), c->st));
+    } else if (c->mixed) {
       HIPCHK(fmmdev::launch_m2l_mfma32(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
                                        c->g_sc.as<double2>(), c->g_tgt.as<int>(), c->g_src.as<int>(),
                                        c->M32.as<float2>(), c->L.as<double2>(), c->st));

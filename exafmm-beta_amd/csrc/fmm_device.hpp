@@ -52,6 +52,17 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
                       const double2 *L, hipStream_t s);
 // P2P for every target leaf over its CSR list (laplace.h:150-235); overwrites
 // the targets of each listed leaf
+// target-stationary grouped M2L (tiles of m2l_tile_targets(P) targets, slices
+// of <= 16 pairs of one group, one block per work item of a tile); mix: FP32
+// operands on M32, else FP64 on M.  A tile in one item stores its targets' L,
+// a split tile adds into L (which must start at zero).
+int m2l_tile_targets(int P);
+hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const int4 *items, const int4 *slices,
+                           const int *psrc, const unsigned char *plt, const double2 *gY, const double2 *gscale,
+                           const double *cr, const void *M, double2 *L, hipStream_t s);
+// singular harmonics of every group offset (group_harmonics_size(P) complex each)
+int group_harmonics_size(int P);
+hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 *Y, hipStream_t s);
 // mixed precision (FP32 list phases): M32 = M~ / R^n per cell; grouped FP32-MFMA
 // M2L with per-group scales (R_s, R_t) accumulating into the FP64 L; FP32 P2P
 bool m2l_mixed_supported(int P);

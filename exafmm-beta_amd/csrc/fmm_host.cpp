@@ -648,7 +648,102 @@ void deal_m2l_chunks(M2LGroups &G, int chunk_pairs) {
     }
 }
 
-void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs) {
+void split_m2l_tiles(M2LTiles &T, int seg) {
+  T.item.clear();
+  for (int b = 0; b < T.ntile; b++) {
+    const int a = T.ptr[b], e = T.ptr[b + 1];
+    const int split = e - a > seg ? 1 : 0;
+    for (int k = a; k < e; k += seg) T.item.insert(T.item.end(), {b, k, std::min(e, k + seg), split});
+  }
+}
+
+M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int seg) {
+  M2LTiles T;
+  T.TT = TT;
+  const int nc = t.ncell;
+  const int64_t np = (int64_t)G.tgt.size();
+  T.ptr.assign(1, 0);
+  if (np == 0) return T;
+  // targets of grouped pairs, ordered by (level, octant in parent, cell)
+  std::vector<char> has(nc, 0);
+  for (int c : G.tgt) has[c] = 1;
+  std::vector<int> tg;
+  for (int c = 0; c < nc; c++)
+    if (has[c]) tg.push_back(c);
+  auto octant = [&](int c) {
+    if (c == 0) return 0;
+    const int p = t.iparent[c];
+    int o = 0;
+    for (int d = 0; d < 3; d++)
+      if (t.X[3 * (size_t)c + d] > t.X[3 * (size_t)p + d]) o |= 1 << d;
+    return o;
+  };
+  std::vector<int64_t> key(tg.size());
+  for (size_t i = 0; i < tg.size(); i++)
+    key[i] = ((int64_t)t.level[tg[i]] << 40) | ((int64_t)octant(tg[i]) << 32) | tg[i];
+  std::vector<int> ord(tg.size());
+  for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  const int nt = (int)((tg.size() + TT - 1) / TT);
+  std::vector<int> tile_of(nc, -1), local(nc, -1);
+  for (size_t i = 0; i < ord.size(); i++) {
+    const int c = tg[ord[i]];
+    tile_of[c] = (int)(i / TT);
+    local[c] = (int)(i % TT);
+  }
+  // pairs bucketed by tile; taking them in group order keeps each tile's
+  // pairs sorted by group
+  std::vector<int64_t> pcount(nt + 1, 0);
+  for (int64_t q = 0; q < np; q++) pcount[tile_of[G.tgt[q]] + 1]++;
+  for (int i = 0; i < nt; i++) pcount[i + 1] += pcount[i];
+  std::vector<int> pgrp(np), psrc(np);
+  std::vector<uint8_t> plt(np);
+  {
+    std::vector<int64_t> fill(pcount.begin(), pcount.end() - 1);
+    for (int g = 0; g < G.ngroup; g++)
+      for (int q = G.grp_ptr[g]; q < G.grp_ptr[g + 1]; q++) {
+        const int c = G.tgt[q];
+        const int64_t k = fill[tile_of[c]]++;
+        pgrp[k] = g;
+        psrc[k] = G.src[q];
+        plt[k] = (uint8_t)local[c];
+      }
+  }
+  // launch order: XCD x (block b runs on XCD b % 8) takes the x-th run of
+  // consecutive tiles, so concurrent tiles on one XCD are neighbours
+  const int kXcd = 8;
+  const int per = (nt + kXcd - 1) / kXcd;
+  T.ntile = per * kXcd;
+  T.tgt.assign((size_t)T.ntile * TT, -1);
+  T.src.reserve(np);
+  T.lt.reserve(np);
+  for (int b = 0; b < T.ntile; b++) {
+    const int tile = (b % kXcd) * per + b / kXcd;
+    if (tile < nt) {
+      for (int j = 0; j < TT; j++) {
+        const size_t i = (size_t)tile * TT + j;
+        if (i < ord.size()) T.tgt[(size_t)b * TT + j] = tg[ord[i]];
+      }
+      for (int64_t k = pcount[tile]; k < pcount[tile + 1];) {
+        const int g = pgrp[k];
+        int64_t e = k;
+        while (e < pcount[tile + 1] && pgrp[e] == g && e - k < SW) e++;
+        T.slice.insert(T.slice.end(), {g, (int)T.src.size(), (int)(e - k), 0});
+        for (int64_t u = k; u < e; u++) {
+          T.src.push_back(psrc[u]);
+          T.lt.push_back(plt[u]);
+        }
+        k = e;
+      }
+    }
+    T.ptr.push_back((int)(T.slice.size() / 4));
+  }
+  split_m2l_tiles(T, seg);
+  return T;
+}
+
+void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs,
+                 M2LTiles *tiles) {
   T = Tree();
   L = Lists();
   int64_t nb = 0, nc = 0, npm = 0, npp = 0;
@@ -725,6 +820,32 @@ void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroup
   }
   G->nrest = (int64_t)G->rest_src.size();
   deal_m2l_chunks(*G, chunk_pairs);
+  if (!tiles) return;
+  // tiles concatenated part by part (each part's tile count is a multiple of
+  // the XCD count, so every part keeps its XCD dealing)
+  M2LTiles &X = *tiles;
+  X = M2LTiles();
+  X.TT = parts[0].tl->TT;
+  X.ptr.assign(1, 0);
+  co = 0;
+  int go = 0;
+  for (const PlanPart &p : parts) {
+    const M2LTiles &a = *p.tl;
+    const int so = (int)(X.slice.size() / 4), po2 = (int)X.src.size();
+    for (int v : a.tgt) X.tgt.push_back(v >= 0 ? v + co : -1);
+    for (int b = 0; b < a.ntile; b++) X.ptr.push_back(a.ptr[b + 1] + so);
+    for (size_t q = 0; q < a.slice.size(); q += 4)
+      X.slice.insert(X.slice.end(), {a.slice[q] + go, a.slice[q + 1] + po2, a.slice[q + 2], 0});
+    for (int v : a.src) X.src.push_back(v + co);
+    X.lt.insert(X.lt.end(), a.lt.begin(), a.lt.end());
+    X.ntile += a.ntile;
+    co += p.t->ncell;
+    go += p.g->ngroup;
+  }
+  int seg = 1;
+  for (const PlanPart &p : parts)
+    for (size_t q = 0; q < p.tl->item.size(); q += 4) seg = std::max(seg, p.tl->item[q + 2] - p.tl->item[q + 1]);
+  split_m2l_tiles(X, seg);
 }
 
 namespace {

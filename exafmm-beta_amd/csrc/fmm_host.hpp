@@ -82,6 +82,28 @@ struct M2LGroups {
 // operator depends on R_s and R_t); pairs whose radii differ from their
 // group's are left ungrouped.
 M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level = false);
+// Target-stationary layout of the grouped M2L pairs: targets are sorted by
+// (level, octant within the parent, cell) -- cells of one octant class share
+// their offsets -- and cut into tiles of TT targets.  A tile's pairs are
+// ordered by group and cut into slices of at most SW pairs of one group, so a
+// block can apply each group's operator to whole slices and keep the tile's
+// local expansions in LDS until one final store.  Tiles are listed in launch
+// order (dealt to the 8 XCDs in contiguous runs; empty tiles pad).
+// A block runs one work item: at most `seg` consecutive slices of one tile.
+// A tile split over several items adds its results (atomics); a tile in one
+// item stores them.
+struct M2LTiles {
+  int TT = 0, ntile = 0;
+  std::vector<int> tgt;        // ntile*TT target cells (-1 pads), launch order
+  std::vector<int> ptr;        // ntile+1 offsets into the slices
+  std::vector<int> slice;      // 4 per slice: group, first pair, count, 0
+  std::vector<int> src;        // source cell of every pair
+  std::vector<uint8_t> lt;     // target of every pair as its index in the tile
+  std::vector<int> item;       // 4 per work item: tile, first slice, end slice, split (0/1)
+};
+M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int seg = 64);
+void split_m2l_tiles(M2LTiles &T, int seg);
+
 // (Re)build G.chunk from the groups: chunks of at most chunk_pairs pairs,
 // ordered by first target and dealt round-robin-contiguously to the 8 XCDs.
 void deal_m2l_chunks(M2LGroups &G, int chunk_pairs);
@@ -94,8 +116,12 @@ struct PlanPart {
   const Tree *t;
   const Lists *l;
   const M2LGroups *g;
+  const M2LTiles *tl;
 };
-void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs);
+// tiles (optional): the parts' tiles concatenated (every part must have tiles
+// of the same TT)
+void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroups *G, int chunk_pairs,
+                 M2LTiles *tiles = nullptr);
 
 // Dataset generators (include/dataset.h:52-171, 220-227): drand48 streams
 // seeded with srand48(seed), positions first, then charges after reseeding,

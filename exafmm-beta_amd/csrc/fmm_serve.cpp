@@ -13,9 +13,9 @@
 // Requests arriving while others run are planned at once and join the next
 // batch a slot takes: batches form continuously, and batches of different
 // slots overlap on the device (separate streams, default priority).  A
-// system's arithmetic does not depend on its batch: its cells, lists and M2L
-// chunks are the same as alone, only the order of the FP64 atomic additions
-// into L differs.
+// system's arithmetic does not depend on its batch: its cells, lists, M2L
+// tiles and chunks are the same as alone; only the order of FP64 atomic
+// additions into L may differ.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -92,7 +92,7 @@ void builder_loop(fmm_server *s) {
     }
     fmm_plan *pl = nullptr;
     int rc = fmm_plan_build(&pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn);
-    if (rc == FMM_OK && s->p.m2l_mode == FMM_M2L_AUTO) rc = fmm_plan_build_groups(pl, s->p.precision);
+    if (rc == FMM_OK) rc = fmm_plan_prepare(pl, &s->p);
     fmm_stats st{};
     if (rc == FMM_OK) fmm_plan_stats(pl, &st);
     std::lock_guard<std::mutex> lk(s->mu);

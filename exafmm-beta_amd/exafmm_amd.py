@@ -27,13 +27,14 @@ SYMBOLS = [
     "fmm_plan_tree", "fmm_plan_lists", "fmm_make_bodies",
     "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
-    "fmm_server_get_stats", "fmm_server_destroy",
+    "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
 ]
 
 FMM_FP64 = 0
 FMM_MIXED = 1
-M2L_AUTO = 0      # offset-grouped FP64 MFMA where supported (P <= 10)
-M2L_TARGET = 1    # per-target kernel, fixed summation order
+M2L_AUTO = 0      # offset groups on MFMA, target-stationary tiles (P <= 12), fixed order
+M2L_TARGET = 1    # per-target kernel on vector ALUs, fixed summation order
+M2L_GROUPED = 2   # offset groups on MFMA, one block per group chunk, FP64 atomics
 
 
 class ServerConfig(ctypes.Structure):
@@ -134,6 +135,7 @@ def load():
                                          _ip, _ip, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_char_p, ctypes.c_int]
         L.fmm_plan_build_groups.argtypes = [vp, ctypes.c_int]
+        L.fmm_plan_prepare.argtypes = [vp, ctypes.POINTER(Params)]
         L.fmm_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
         L.fmm_server_default_config.restype = None
         L.fmm_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params),
@@ -355,6 +357,17 @@ class Plan:
         if rc != 0:
             raise FMMError("fmm_plan_merge failed (rc=%d)" % rc)
         return self
+
+    def prepare(self, P=10, precision=FMM_FP64, m2l_mode=M2L_AUTO):
+        """Build the M2L groups (and target tiles) a context with these
+        parameters needs (fmm_plan_prepare)."""
+        L = load()
+        p = Params()
+        L.fmm_default_params(ctypes.byref(p))
+        p.P, p.precision, p.m2l_mode = P, precision, m2l_mode
+        rc = L.fmm_plan_prepare(self._h, ctypes.byref(p))
+        if rc != 0:
+            raise FMMError("fmm_plan_prepare failed (rc=%d)" % rc)
 
     def build_groups(self, precision=FMM_FP64):
         rc = load().fmm_plan_build_groups(self._h, precision)

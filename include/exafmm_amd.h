@@ -36,13 +36,19 @@ typedef enum {
  * added into FP64 accumulators; M2L as grouped FP32 MFMA translations whose
  * results are added into FP64 local expansions); P2M, M2M, L2L, L2P, the
  * harmonics of every translation and all accumulation across lists stay FP64.
- * Needs P <= 12 and m2l_mode AUTO. */
+ * Needs P <= 12 and m2l_mode AUTO or GROUPED. */
 typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
 
-/* M2L execution: AUTO uses the offset-grouped FP64 MFMA kernel where the order
- * allows (P <= 10) and the per-target kernel otherwise; TARGET always uses the
- * per-target kernel, whose summation order is fixed (bitwise repeatable). */
-typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1 } fmm_m2l_mode;
+/* M2L execution.  AUTO (P <= 12): pairs grouped by offset (one dense
+ * operator per group, on MFMA) and processed target-stationary -- a block owns
+ * a tile of target cells and sums its groups in LDS before one store (a tile
+ * too long for one block is split, and its parts are added atomically:
+ * repeatable to rounding).  GROUPED
+ * (P <= 12): the same operators, one block per chunk of one group, results
+ * added with FP64 atomics (repeatable to rounding).  TARGET: one wavefront per
+ * target cell on vector ALUs (fixed order; any P).  Pairs whose offset does
+ * not fit a group run per target in every mode. */
+typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1, FMM_M2L_GROUPED = 2 } fmm_m2l_mode;
 
 /* The knobs of exafmm-beta's Kernel / Traversal / BuildTree constructors and
  * Args defaults (include/args.h:188-205): P=10, theta=0.4, ncrit=64,
@@ -142,6 +148,11 @@ int fmm_plan_from_tree(fmm_plan **out, int n, const double *x, int ncell, const 
  * context given the plan does no grouping work itself.  Host only. */
 int fmm_plan_build_groups(fmm_plan *pl, int precision);
 
+/* Everything a context with parameters p needs from the host for its M2L
+ * (groups and, for m2l_mode AUTO, the target tiles), built into the plan so
+ * that fmm_set_bodies_plan only uploads.  Host only. */
+int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p);
+
 /* Batched evaluation of independent systems: one plan for the forest of the
  * k plans (cells, bodies and pairs of plan i follow those of plans 0..i-1; no
  * interaction crosses systems).  Bodies of the merged plan are the k input
@@ -194,7 +205,7 @@ int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p
  * device_budget beside the batches in flight, and evaluates them as one
  * forest (one launch per phase for the whole batch).  Batches of different
  * slots overlap on the GPU.  A system's result does not depend on its batch
- * beyond the order of FP64 additions into its local expansions. */
+ * beyond the order of FP64 atomic additions into its local expansions. */
 typedef struct fmm_server fmm_server;
 
 typedef struct {

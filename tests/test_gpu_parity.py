@@ -192,7 +192,8 @@ def test_input_order_and_determinism():
     f.evaluate()
     assert np.array_equal(f.targets(input_order=False), a)
     f.close()
-    # grouped MFMA M2L accumulates with atomics: repeatable to rounding
+    # grouped M2L in target tiles: tiles split over several blocks add their
+    # parts with atomics, so repeatable to rounding
     g = F.FMM(P=8, m2l_mode=F.M2L_AUTO)
     g.set_bodies(x, q)
     g.evaluate()
@@ -202,14 +203,24 @@ def test_input_order_and_determinism():
     assert _rel(c, a) < 1e-13
     assert g.stats()["m2l_path"] == F.M2L_AUTO
     g.close()
+    # grouped chunks accumulate with atomics: repeatable to rounding
+    h = F.FMM(P=8, m2l_mode=F.M2L_GROUPED)
+    h.set_bodies(x, q)
+    h.evaluate()
+    d = h.targets(input_order=False)
+    h.evaluate()
+    assert _rel(h.targets(input_order=False), d) < 1e-14
+    assert _rel(d, a) < 1e-13
+    assert h.stats()["m2l_path"] == F.M2L_GROUPED
+    h.close()
 
 
-@pytest.mark.parametrize("P", [6, 10])
+@pytest.mark.parametrize("P", [6, 8, 10, 12])
 def test_grouped_m2l_matches_per_target(P):
     x, q = _inputs(20000, "plummer", seed=4)
     F = _fmm()
     out = []
-    for mode in (F.M2L_TARGET, F.M2L_AUTO):
+    for mode in (F.M2L_TARGET, F.M2L_AUTO, F.M2L_GROUPED):
         f = F.FMM(P=P, m2l_mode=mode)
         f.set_bodies(x, q)
         f.upward()
@@ -218,7 +229,7 @@ def test_grouped_m2l_matches_per_target(P):
         st = f.stats()
         f.close()
     assert st["m2l_groups"] > 0 and st["m2l_ungrouped"] == 0
-    assert _rel(out[1], out[0]) < PHASE_TOL
+    assert _rel(out[1], out[0]) < PHASE_TOL and _rel(out[2], out[0]) < PHASE_TOL
 
 
 def test_set_charges_linearity():

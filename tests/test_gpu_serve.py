@@ -1,9 +1,9 @@
 """The request loop (fmm_server_*): every result is the same alone and in a
 batch, batching and slot overlap actually happen, and memory admission holds.
 
-Tolerance alone vs in the mix: 1e-12 relative (L2 per request).  A system's
-cells, lists and M2L chunks are identical in both cases; only the order of
-the FP64 atomic additions into its local expansions differs.
+Alone vs in the mix: 1e-12 relative (L2 per request).  A system's cells,
+lists, M2L tiles and chunks are identical in both cases; only the order of
+FP64 atomic additions into its local expansions may differ.
 """
 import numpy as np
 import pytest
@@ -33,8 +33,8 @@ def _systems(F, k, seed):
     return out
 
 
-def _serve(F, systems, precision, **cfg):
-    s = F.Server(P=8, precision=precision, **cfg)
+def _serve(F, systems, precision, m2l_mode=0, **cfg):
+    s = F.Server(P=8, precision=precision, m2l_mode=m2l_mode, **cfg)
     ids = [s.submit(x, q) for x, q in systems]
     res = [s.wait(i) for i in ids]
     st = s.stats()
@@ -42,13 +42,13 @@ def _serve(F, systems, precision, **cfg):
     return res, st
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_results_equal_alone_and_in_mix(precision):
+@pytest.mark.parametrize("precision,mode", [("fp64", 0), ("mixed", 0), ("mixed", 2)])
+def test_results_equal_alone_and_in_mix(precision, mode):
     F = _F()
     prec = F.FMM_MIXED if precision == "mixed" else F.FMM_FP64
     systems = _systems(F, 12, 1)
-    alone, st1 = _serve(F, systems, prec, slots=1, batch_requests=1)
-    mix, st2 = _serve(F, systems, prec, slots=3, batch_requests=64, batch_delay_ms=500.0)
+    alone, st1 = _serve(F, systems, prec, slots=1, batch_requests=1, m2l_mode=mode)
+    mix, st2 = _serve(F, systems, prec, slots=3, batch_requests=64, batch_delay_ms=500.0, m2l_mode=mode)
     assert st1["max_batch_requests"] == 1 and st1["batches"] == 12
     assert st2["max_batch_requests"] > 1 and st2["requests_done"] == 12
     for (ta, ia), (tm, im), (x, q) in zip(alone, mix, systems):
@@ -56,7 +56,7 @@ def test_results_equal_alone_and_in_mix(precision):
         assert _rel(tm[:, 0], ta[:, 0]) < MIX_TOL and _rel(tm[:, 1:], ta[:, 1:]) < MIX_TOL
     # and equal to a context solving the system by itself
     for (ta, _), (x, q) in zip(alone[:3], systems[:3]):
-        f = F.FMM(P=8, precision=prec)
+        f = F.FMM(P=8, precision=prec, m2l_mode=mode)
         f.set_bodies(x, q)
         f.evaluate()
         t = f.targets(input_order=True)

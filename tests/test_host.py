@@ -157,3 +157,16 @@ def test_server_without_device_reports_error():
         F.Server(P=8)
     with pytest.raises(F.FMMError):
         F.Server(P=1)
+
+
+@pytest.mark.parametrize("n,tt", [(20000, 32), (60000, 64)])
+def test_m2l_tiles_cover_every_pair(tmp_path, n, tt):
+    """tests/cpp/tiles_check.cpp against the package's host objects."""
+    import subprocess
+    pkg = os.path.join(ROOT, "exafmm-beta_amd")
+    exe = str(tmp_path / "tiles_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(pkg, "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "tiles_check.cpp"),
+                    os.path.join(pkg, "build", "fmm_host.o"), "-o", exe], check=True)
+    out = subprocess.run([exe, str(n), str(tt)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
