@@ -328,7 +328,7 @@ int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
       pl->tiles = fmm::M2LTiles();
     }
     const int TT = fmmdev::m2l_tile_targets(p->P);
-    if (p->m2l_mode == FMM_M2L_AUTO && pl->tiles.TT != TT)
+    if (p->m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(p->P) && pl->tiles.TT != TT)
       pl->tiles = fmm::build_m2l_tiles(pl->tree, pl->groups, TT, 16);
   } catch (const std::bad_alloc &) {
     return FMM_ERR_MEMORY;
@@ -503,7 +503,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   c->mixed = c->p.precision == FMM_MIXED;
   c->use_mfma = c->p.m2l_mode != FMM_M2L_TARGET && L.nm2l > 0 &&
                 (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
-  c->use_tiles = c->use_mfma && c->p.m2l_mode == FMM_M2L_AUTO;
+  c->use_tiles = c->use_mfma && c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(c->p.P);
   c->groups = fmm::M2LGroups();
   c->nchunk = 0;
   c->ntile = 0;

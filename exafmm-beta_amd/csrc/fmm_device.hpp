@@ -57,6 +57,7 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
 // operands on M32, else FP64 on M.  A tile in one item stores its targets' L,
 // a split tile adds into L (which must start at zero).
 int m2l_tile_targets(int P);
+bool m2l_tiles_preferred(int P);  // AUTO uses tiles (else chunks) at this order
 hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const int4 *items, const int4 *slices,
                            const int *psrc, const unsigned char *plt, const double2 *gY, const double2 *gscale,
                            const double *cr, const void *M, double2 *L, hipStream_t s);

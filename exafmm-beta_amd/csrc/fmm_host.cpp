@@ -497,6 +497,47 @@ struct KeyIndex {
   }
 };
 
+// Open-addressing index of 128-bit keys (offset key, level pair) -> dense id;
+// one 24-byte slot per probe.
+struct KeyIndex2 {
+  struct Slot {
+    uint64_t a, b;
+    int id;
+  };
+  std::vector<Slot> t;
+  size_t mask = 0;
+  int count = 0;
+  explicit KeyIndex2(size_t cap = 1 << 12) { rehash(cap); }
+  void rehash(size_t cap) {
+    std::vector<Slot> o;
+    o.swap(t);
+    t.assign(cap, Slot{0, 0, -1});
+    mask = cap - 1;
+    for (const Slot &x : o)
+      if (x.id >= 0) place(x);
+  }
+  static inline size_t hash(uint64_t a, uint64_t b) { return KeyIndex::mix(a ^ (b * 0x9e3779b97f4a7c15ull)); }
+  void place(const Slot &x) {
+    size_t h = hash(x.a, x.b) & mask;
+    while (t[h].id >= 0) h = (h + 1) & mask;
+    t[h] = x;
+  }
+  int get(uint64_t a, uint64_t b, bool &fresh) {
+    size_t h = hash(a, b) & mask;
+    while (t[h].id >= 0) {
+      if (t[h].a == a && t[h].b == b) {
+        fresh = false;
+        return t[h].id;
+      }
+      h = (h + 1) & mask;
+    }
+    fresh = true;
+    t[h] = Slot{a, b, count++};
+    if (2 * (size_t)count > t.size()) rehash(2 * t.size());
+    return count - 1;
+  }
+};
+
 }  // namespace
 
 M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level) {
@@ -510,22 +551,33 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   const double R0 = t.R[0];
   const int64_t np = L.nm2l;
   std::vector<int> gid(np);
-  std::vector<int64_t> kvec;  // 3 per group
-  std::vector<int> gexp;      // e per group
-  KeyIndex index;
+  std::vector<int64_t> kvec;   // 3 per group
+  std::vector<int> gexp;       // e per group
+  std::vector<double> sc;      // by_level: (R_s, R_t) per group
+  KeyIndex2 index;
   const int64_t lim = 1 << 18;
+  double pow2[64];             // 2^e / R0
+  for (int e = 0; e < 64; e++) pow2[e] = std::ldexp(1.0, e) / R0;
+  // round to nearest (ties to even) for |v| < 2^51 without a libm call
+  auto rnd = [](double v) { return (v + 6755399441055744.0) - 6755399441055744.0; };
   for (int c = 0; c < nc; c++) {
+    const double xc = t.X[3 * (size_t)c], yc = t.X[3 * (size_t)c + 1], zc = t.X[3 * (size_t)c + 2];
+    const int lt = t.level[c];
     for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
-      const int sc = L.m2l_src[p];
-      int e = std::max(t.level[c], t.level[sc]);
-      const double scale = std::ldexp(1.0, e) / R0;
+      const int scell = L.m2l_src[p];
+      const int ls = t.level[scell];
+      int e = std::max(lt, ls);
       int64_t k[3];
       bool ok = e < 64;
-      for (int d = 0; d < 3 && ok; d++) {
-        const double v = (t.X[3 * (size_t)c + d] - t.X[3 * (size_t)sc + d]) * scale;
-        const double r = std::nearbyint(v);
-        if (!(std::fabs(v - r) < 1e-6) || std::fabs(r) >= (double)(lim << 8)) ok = false;
-        k[d] = (int64_t)r;
+      if (ok) {
+        const double scale = pow2[e];
+        const double v[3] = {(xc - t.X[3 * (size_t)scell]) * scale, (yc - t.X[3 * (size_t)scell + 1]) * scale,
+                             (zc - t.X[3 * (size_t)scell + 2]) * scale};
+        for (int d = 0; d < 3; d++) {
+          const double r = rnd(v[d]);
+          if (!(std::fabs(v[d] - r) < 1e-6) || std::fabs(r) >= (double)(lim << 8)) ok = false;
+          k[d] = (int64_t)r;
+        }
       }
       if (ok) {
         while (e > 0 && ((k[0] | k[1] | k[2]) & 1) == 0) {
@@ -543,53 +595,29 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
       }
       const uint64_t key = ((uint64_t)e << 57) | ((uint64_t)(k[0] + lim) << 38) |
                            ((uint64_t)(k[1] + lim) << 19) | (uint64_t)(k[2] + lim);
+      // by_level: the source and target levels are part of the group
+      const uint64_t key2 = by_level ? (((uint64_t)lt << 8) | (uint64_t)ls) + 1 : 0;
       bool fresh;
-      const int g = index.get(key, fresh);
+      const int g = index.get(key, key2, fresh);
       if (fresh) {
         kvec.insert(kvec.end(), {k[0], k[1], k[2]});
         gexp.push_back(e);
+        if (by_level) sc.insert(sc.end(), {t.R[scell], t.R[c]});
+      }
+      if (by_level && (t.R[scell] != sc[2 * (size_t)g] || t.R[c] != sc[2 * (size_t)g + 1])) {
+        gid[p] = -1;  // radius not uniform on its level: FP64 per-target path
+        continue;
       }
       gid[p] = g;
     }
   }
-  int ng = index.count;
-  std::vector<double> odx(3 * (size_t)ng);
+  const int ng = index.count;
+  G.dx.resize(3 * (size_t)ng);
   for (int g = 0; g < ng; g++) {
     const double u = R0 / std::ldexp(1.0, gexp[g]);
-    for (int d = 0; d < 3; d++) odx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
+    for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
   }
-  if (by_level) {
-    // split each offset group by (target level, source level)
-    KeyIndex lindex;
-    std::vector<int> base;  // offset group of each split group
-    std::vector<double> sc;
-    for (int c = 0; c < nc; c++) {
-      for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
-        if (gid[p] < 0) continue;
-        const int sc_cell = L.m2l_src[p];
-        const int lt = t.level[c], ls = t.level[sc_cell];
-        const uint64_t key = ((uint64_t)gid[p] << 12) | ((uint64_t)lt << 6) | (uint64_t)ls;
-        bool fresh;
-        const int g2 = lindex.get(key, fresh);
-        if (fresh) {
-          base.push_back(gid[p]);
-          sc.insert(sc.end(), {t.R[sc_cell], t.R[c]});
-        }
-        if (t.R[sc_cell] != sc[2 * (size_t)g2] || t.R[c] != sc[2 * (size_t)g2 + 1]) {
-          gid[p] = -1;  // radius not uniform on its level: FP64 per-target path
-          continue;
-        }
-        gid[p] = g2;
-      }
-    }
-    ng = lindex.count;
-    G.dx.resize(3 * (size_t)ng);
-    for (int g = 0; g < ng; g++)
-      for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = odx[3 * (size_t)base[g] + d];
-    G.scale = std::move(sc);
-  } else {
-    G.dx = std::move(odx);
-  }
+  if (by_level) G.scale = std::move(sc);
   G.ngroup = ng;
   // counting sort by group, CSR order kept inside a group
   G.grp_ptr.assign(ng + 1, 0);

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "exafmm_amd.h"
+#include "fmm_host.hpp"
 
 namespace {
 
@@ -81,6 +82,7 @@ int64_t footprint(const fmm_params &p, const fmm_stats &s) {
 }
 
 void builder_loop(fmm_server *s) {
+  fmm::set_host_threads(1);  // requests are planned in parallel, each on one thread
   for (;;) {
     std::shared_ptr<Request> r;
     {

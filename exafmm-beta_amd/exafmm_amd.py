@@ -32,7 +32,7 @@ SYMBOLS = [
 
 FMM_FP64 = 0
 FMM_MIXED = 1
-M2L_AUTO = 0      # offset groups on MFMA, target-stationary tiles (P <= 12), fixed order
+M2L_AUTO = 0      # offset groups on MFMA: target tiles for P <= 8, chunks for 9 <= P <= 12
 M2L_TARGET = 1    # per-target kernel on vector ALUs, fixed summation order
 M2L_GROUPED = 2   # offset groups on MFMA, one block per group chunk, FP64 atomics
 

@@ -39,15 +39,16 @@ typedef enum {
  * Needs P <= 12 and m2l_mode AUTO or GROUPED. */
 typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
 
-/* M2L execution.  AUTO (P <= 12): pairs grouped by offset (one dense
- * operator per group, on MFMA) and processed target-stationary -- a block owns
- * a tile of target cells and sums its groups in LDS before one store (a tile
- * too long for one block is split, and its parts are added atomically:
- * repeatable to rounding).  GROUPED
- * (P <= 12): the same operators, one block per chunk of one group, results
- * added with FP64 atomics (repeatable to rounding).  TARGET: one wavefront per
- * target cell on vector ALUs (fixed order; any P).  Pairs whose offset does
- * not fit a group run per target in every mode. */
+/* M2L execution.  Pairs are grouped by offset (one dense operator per group,
+ * applied on MFMA) in two layouts.  Target tiles: a block owns a tile of
+ * target cells and sums all their groups in LDS before one store (a tile too
+ * long for one block is split and its parts are added atomically).  Chunks:
+ * one block per chunk of <= 512 pairs of one group, results added with FP64
+ * atomics.  AUTO (P <= 12) uses tiles for P <= 8 and chunks above (measured:
+ * atomics dominate at low order, operator builds at high order); GROUPED
+ * always uses chunks; both repeat to rounding.  TARGET: one wavefront per
+ * target cell on vector ALUs (fixed summation order, bitwise repeatable; any
+ * P).  Pairs whose offset does not fit a group run per target in every mode. */
 typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1, FMM_M2L_GROUPED = 2 } fmm_m2l_mode;
 
 /* The knobs of exafmm-beta's Kernel / Traversal / BuildTree constructors and
