@@ -677,12 +677,26 @@ void deal_m2l_chunks(M2LGroups &G, int chunk_pairs) {
 }
 
 void split_m2l_tiles(M2LTiles &T, int seg) {
-  T.item.clear();
+  std::vector<int> it;
   for (int b = 0; b < T.ntile; b++) {
     const int a = T.ptr[b], e = T.ptr[b + 1];
     const int split = e - a > seg ? 1 : 0;
-    for (int k = a; k < e; k += seg) T.item.insert(T.item.end(), {b, k, std::min(e, k + seg), split});
+    for (int k = a; k < e; k += seg) it.insert(it.end(), {b, k, std::min(e, k + seg), split});
   }
+  // launch order: block i runs on XCD i % 8.  Runs of kRun consecutive items
+  // (neighbouring tiles, sharing source multipoles) go to one XCD, and the
+  // runs are dealt round-robin so every XCD gets a similar mix of work;
+  // empty items (s0 == s1) pad the shorter queues
+  const int kXcd = 8, kRun = 16;
+  const size_t n = it.size() / 4;
+  std::vector<std::vector<size_t>> q(kXcd);
+  for (size_t i = 0; i < n; i++) q[(i / kRun) % kXcd].push_back(i);
+  size_t per = 0;
+  for (auto &v : q) per = std::max(per, v.size());
+  T.item.assign(4 * per * kXcd, 0);
+  for (int x = 0; x < kXcd; x++)
+    for (size_t j = 0; j < q[x].size(); j++)
+      for (int k = 0; k < 4; k++) T.item[4 * (j * kXcd + x) + k] = it[4 * q[x][j] + k];
 }
 
 M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int seg) {
@@ -737,17 +751,15 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
         plt[k] = (uint8_t)local[c];
       }
   }
-  // launch order: XCD x (block b runs on XCD b % 8) takes the x-th run of
-  // consecutive tiles, so concurrent tiles on one XCD are neighbours
-  const int kXcd = 8;
-  const int per = (nt + kXcd - 1) / kXcd;
-  T.ntile = per * kXcd;
+  // tiles in key order (neighbours adjacent); the work items built from them
+  // are dealt to the XCDs by split_m2l_tiles
+  T.ntile = nt;
   T.tgt.assign((size_t)T.ntile * TT, -1);
   T.src.reserve(np);
   T.lt.reserve(np);
   for (int b = 0; b < T.ntile; b++) {
-    const int tile = (b % kXcd) * per + b / kXcd;
-    if (tile < nt) {
+    const int tile = b;
+    {
       for (int j = 0; j < TT; j++) {
         const size_t i = (size_t)tile * TT + j;
         if (i < ord.size()) T.tgt[(size_t)b * TT + j] = tg[ord[i]];

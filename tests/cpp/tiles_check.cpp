@@ -3,6 +3,8 @@
 // owns its target; a slice holds at most 16 pairs of one group with distinct
 // targets; a tile's targets are distinct and every target has one tile.
 // Prints "ok" or the first violation.
+#include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -45,8 +47,13 @@ static int check(const Tree &t, const M2LGroups &G, const M2LTiles &T) {
   // work items cover every tile's slices once, in order
   std::vector<int> next(T.ntile);
   for (int b = 0; b < T.ntile; b++) next[b] = T.ptr[b];
-  for (size_t k = 0; k < T.item.size(); k += 4) {
-    const int b = T.item[k], a = T.item[k + 1], e = T.item[k + 2], split = T.item[k + 3];
+  // items in tile order (the launch order deals them to the XCDs)
+  std::vector<std::array<int, 4>> items;
+  for (size_t k = 0; k < T.item.size(); k += 4)
+    if (T.item[k + 2] > T.item[k + 1]) items.push_back({T.item[k], T.item[k + 1], T.item[k + 2], T.item[k + 3]});
+  std::sort(items.begin(), items.end());
+  for (const auto &itm : items) {
+    const int b = itm[0], a = itm[1], e = itm[2], split = itm[3];
     if (a != next[b] || e <= a || e > T.ptr[b + 1]) return printf("bad work item\n"), 1;
     if (split != ((a != T.ptr[b] || e != T.ptr[b + 1]) ? 1 : 0)) return printf("bad split flag\n"), 1;
     next[b] = e;
