@@ -61,6 +61,10 @@ struct fmm_ctx {
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
   DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y;
+  // M2M / L2L as per-level operators on MFMA (regular trees)
+  bool xl = false;
+  std::vector<int> xl_tptr;  // per child level: first L2L tile
+  DBuf xl_child8, xl_children, xl_tiles, xl_ops, xl_dx, xl_kinds;
   // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
   bool mixed = false;
   DBuf g_sc, cr, M32;
@@ -500,6 +504,66 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
   UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
+  // M2M / L2L operators: every level's radius uniform and every child at
+  // X_parent + R_child (+-1, +-1, +-1) (up to rounding of the centres)
+  c->xl = fmmdev::xlate_mfma_supported(c->p.P) && nc > 1;
+  std::vector<double> lrad(t.depth + 1, -1.0);
+  for (int k = 0; k < nc && c->xl; k++) {
+    const int l = t.level[k];
+    if (lrad[l] < 0) lrad[l] = t.R[k];
+    if (t.R[k] != lrad[l]) c->xl = false;
+    if (k > 0)
+      for (int d = 0; d < 3 && c->xl; d++) {
+        const double off = std::fabs(t.X[3 * (size_t)k + d] - t.X[3 * (size_t)t.iparent[k] + d]);
+        if (!(std::fabs(off - t.R[k]) <= 1e-9 * t.R[k])) c->xl = false;
+      }
+  }
+  if (c->xl) {
+    auto octant = [&](int k) {
+      int o = 0;
+      for (int d = 0; d < 3; d++)
+        if (t.X[3 * (size_t)k + d] > t.X[3 * (size_t)t.iparent[k] + d]) o |= 1 << d;
+      return o;
+    };
+    std::vector<int> child8(8 * internal.size(), -1);
+    for (size_t i = 0; i < internal.size(); i++) {
+      const int pc = internal[i];
+      for (int k = t.ichild[pc]; k < t.ichild[pc] + t.nchild[pc]; k++) child8[8 * i + octant(k)] = k;
+    }
+    // L2L tiles: each level's children grouped by octant, 16 per tile
+    std::vector<int> children, tiles;
+    c->xl_tptr.assign(t.depth + 2, 0);
+    for (int l = 1; l <= t.depth; l++) {
+      c->xl_tptr[l] = (int)(tiles.size() / 4);
+      for (int o = 0; o < 8; o++) {
+        const int first = (int)children.size();
+        for (int k = t.level_ptr[l]; k < t.level_ptr[l + 1]; k++) {
+          const int cell = t.level_cells[k];
+          if (octant(cell) == o) children.push_back(cell);
+        }
+        for (int b = first; b < (int)children.size(); b += 16)
+          tiles.insert(tiles.end(), {o, b, std::min(16, (int)children.size() - b), 0});
+      }
+    }
+    c->xl_tptr[t.depth + 1] = (int)(tiles.size() / 4);
+    // operators: M2M for parent levels 0..depth-1, L2L for child levels 1..depth
+    const int nop = 16 * (t.depth + 1);
+    std::vector<double> dxs(3 * (size_t)nop, 0.0);
+    std::vector<int> kinds(nop, 0);
+    for (int l = 0; l <= t.depth; l++)
+      for (int o = 0; o < 8; o++) {
+        const int m = l * 8 + o, q = 8 * (t.depth + 1) + l * 8 + o;
+        kinds[q] = 1;
+        for (int d = 0; d < 3; d++) {
+          const double s = (o >> d) & 1 ? 1.0 : -1.0;
+          if (l < t.depth && lrad[l + 1] > 0) dxs[3 * (size_t)m + d] = -s * lrad[l + 1];  // X_parent - X_child
+          if (lrad[l] > 0) dxs[3 * (size_t)q + d] = s * lrad[l];                         // X_child - X_parent
+        }
+        if (l == t.depth || lrad[l + 1] < 0) dxs[3 * (size_t)m] = 1.0;  // unused operator: any finite offset
+        if (l == 0) dxs[3 * (size_t)q] = 1.0;
+      }
+    UP(xl_child8, child8) UP(xl_children, children) UP(xl_tiles, tiles) UP(xl_dx, dxs) UP(xl_kinds, kinds)
+  }
   c->mixed = c->p.precision == FMM_MIXED;
   c->use_mfma = c->p.m2l_mode != FMM_M2L_TARGET && L.nm2l > 0 &&
                 (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
@@ -539,6 +603,12 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
   if (c->mixed && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
+  if (c->xl) {  // the per-level M2M / L2L operators, once per plan
+    const int nop = 16 * (t.depth + 1);
+    if ((rc = ensure(c, c->xl_ops, sizeof(double) * fmmdev::xlate_op_stride(c->p.P) * nop))) return rc;
+    HIPCHK(fmmdev::launch_xlate_ops(c->p.P, c->xl_dx.as<double>(), c->xl_kinds.as<int>(), nop,
+                                    c->xl_ops.as<double>(), c->st));
+  }
   if (c->use_tiles) {  // harmonics of every group offset, once per plan
     const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * c->groups.ngroup;
     if ((rc = ensure(c, c->g_y, sizeof(double2) * ny))) return rc;
@@ -655,9 +725,13 @@ int fmm_upward(fmm_ctx *c) {
   auto C = dev_cells(c);
   HIPCHK(fmmdev::launch_p2m(P, dev_bodies(c), C, c->leaves.as<int>(), (int)c->tree.leaves.size(),
                             c->M.as<double2>(), c->st));
+  const size_t os = fmmdev::xlate_op_stride(P);
   for (int l = c->tree.depth; l >= 0; l--) {
     const int b = c->internal_ptr[l], e = c->internal_ptr[l + 1];
-    if (e > b)
+    if (e > b && c->xl)
+      HIPCHK(fmmdev::launch_m2m_mfma(P, c->internal.as<int>() + b, c->xl_child8.as<int>() + 8 * (size_t)b, e - b,
+                                     c->xl_ops.as<double>() + os * 8 * l, c->M.as<double2>(), c->st));
+    else if (e > b)
       HIPCHK(fmmdev::launch_m2m(P, C, c->internal.as<int>() + b, e - b, c->M.as<double2>(), c->st));
   }
   return FMM_OK;
@@ -729,10 +803,18 @@ int fmm_downward(fmm_ctx *c) {
   if (c->tree.ncell == 0) return FMM_OK;
   const int P = c->p.P;
   auto C = dev_cells(c);
+  const size_t os = fmmdev::xlate_op_stride(P);
   for (int l = 1; l <= c->tree.depth; l++) {
     const int b = c->level_ptr[l], e = c->level_ptr[l + 1];
-    if (e > b)
+    if (c->xl) {
+      const int t0 = c->xl_tptr[l], t1 = c->xl_tptr[l + 1];
+      if (t1 > t0)
+        HIPCHK(fmmdev::launch_l2l_mfma(P, c->xl_tiles.as<int4>() + t0, t1 - t0, c->xl_children.as<int>(),
+                                       c->iparent.as<int>(), c->xl_ops.as<double>() + os * (8 * (c->tree.depth + 1) + 8 * l),
+                                       c->L.as<double2>(), c->st));
+    } else if (e > b) {
       HIPCHK(fmmdev::launch_l2l(P, C, c->level_cells.as<int>() + b, e - b, c->L.as<double2>(), c->st));
+    }
   }
   HIPCHK(fmmdev::launch_l2p(P, dev_bodies(c), C, c->body_leaf.as<int>(), c->n, c->L.as<double2>(),
                             c->st));

@@ -64,6 +64,18 @@ hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const in
 // singular harmonics of every group offset (group_harmonics_size(P) complex each)
 int group_harmonics_size(int P);
 hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 *Y, hipStream_t s);
+// M2M / L2L as per-level dense operators on MFMA (8 child offsets per level):
+// ops built once per plan (kind 0 M2M with dX = X_parent - X_child, 1 L2L
+// with dX = X_child - X_parent; xlate_op_stride(P) doubles each)
+bool xlate_mfma_supported(int P);
+int xlate_op_stride(int P);
+hipError_t launch_xlate_ops(int P, const double *dxs, const int *kinds, int nop, double *ops, hipStream_t s);
+// parents of one level in tiles of 16; child8[i*8 + octant] (-1: none)
+hipError_t launch_m2m_mfma(int P, const int *parents, const int *child8, int np, const double *ops8, double2 *M,
+                           hipStream_t s);
+// tiles {octant, first, count <= 16} of one level's children grouped by octant
+hipError_t launch_l2l_mfma(int P, const int4 *tiles, int ntile, const int *children, const int *iparent,
+                           const double *ops8, double2 *L, hipStream_t s);
 // mixed precision (FP32 list phases): M32 = M~ / R^n per cell; grouped FP32-MFMA
 // M2L with per-group scales (R_s, R_t) accumulating into the FP64 L; FP32 P2P
 bool m2l_mixed_supported(int P);
