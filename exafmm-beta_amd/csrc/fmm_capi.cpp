@@ -65,6 +65,10 @@ struct fmm_ctx {
   bool xl = false;
   std::vector<int> xl_tptr;  // per child level: first L2L tile
   DBuf xl_child8, xl_children, xl_tiles, xl_ops, xl_dx, xl_kinds;
+  // periodic images: shift unit per image, far-image operator F (images >= 2)
+  double3 cyc = {0.0, 0.0, 0.0};
+  bool far = false;
+  DBuf per_F, per_T, per_S, per_cur, per_tmp, per_dx, per_kind;
   // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
   bool mixed = false;
   DBuf g_sc, cr, M32;
@@ -76,6 +80,8 @@ struct fmm_plan {
   fmm::Lists lists;
   fmm::M2LGroups groups;  // prebuilt M2L groups (fmm_plan_build_groups), kind:
   int groups_kind = -1;   // -1 none, FMM_FP64 offset groups, FMM_MIXED by level
+  int images = 0;          // periodic lists (images > 0) and their cycle
+  double cycle[3] = {0, 0, 0};
   fmm::M2LTiles tiles;    // prebuilt target tiles of those groups (TT > 0)
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   double t_bounds = 0, t_tree = 0, t_lists = 0;
@@ -167,6 +173,7 @@ void fmm_default_params(fmm_params *p) {
   p->precision = FMM_FP64;
   p->device = 0;
   p->m2l_mode = FMM_M2L_AUTO;
+  for (int d = 0; d < 3; d++) p->cycle[d] = 2 * M_PI;  // examples/fmm.cxx:14
 }
 
 int fmm_create(fmm_ctx **out, const fmm_params *p) {
@@ -182,9 +189,9 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
   } else if (p->ncrit < 1 || p->theta <= 0 || p->nspawn < 1) {
     c->err = "invalid ncrit/theta/nspawn";
     rc = FMM_ERR_ARG;
-  } else if (p->images != 0) {
-    c->err = "periodic images are not supported (images must be 0)";
-    rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->images < 0 || (p->images > 0 && !(p->cycle[0] > 0 && p->cycle[1] > 0 && p->cycle[2] > 0))) {
+    c->err = "images must be >= 0, and the cycle > 0 in x, y and z when images > 0";
+    rc = FMM_ERR_ARG;
   } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET && p->m2l_mode != FMM_M2L_GROUPED) {
     c->err = "invalid m2l_mode";
     rc = FMM_ERR_ARG;
@@ -249,7 +256,11 @@ void fmm_destroy(fmm_ctx *c) {
 const char *fmm_last_error(const fmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 namespace {
-void plan_fill(fmm_plan *pl, int n, const double *x, int ncrit, double theta, int nspawn) {
+void plan_fill(fmm_plan *pl, int n, const double *x, int ncrit, double theta, int nspawn, int images = 0,
+               const double *cycle = nullptr) {
+  pl->images = images;
+  if (images > 0)
+    for (int d = 0; d < 3; d++) pl->cycle[d] = cycle[d];
   pl->n = n;
   double t0 = now_ms();
   fmm::get_bounds(n, x, pl->xmin, pl->xmax);
@@ -257,7 +268,7 @@ void plan_fill(fmm_plan *pl, int n, const double *x, int ncrit, double theta, in
   pl->tree = fmm::build_tree(n, x, ncrit, pl->xmin, pl->xmax);
   double t2 = now_ms();
   const double xper[3] = {0, 0, 0};
-  pl->lists = fmm::build_lists(pl->tree, theta, nspawn, xper);
+  pl->lists = fmm::build_lists(pl->tree, theta, nspawn, xper, images, pl->cycle);
   double t3 = now_ms();
   pl->t_bounds = t1 - t0;
   pl->t_tree = t2 - t1;
@@ -311,7 +322,8 @@ void fmm_plan_destroy(fmm_plan *pl) { delete pl; }
 int fmm_plan_build_groups(fmm_plan *pl, int precision) {
   if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED)) return FMM_ERR_ARG;
   try {
-    pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED);
+    pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED,
+                                       pl->images > 0 ? pl->cycle : nullptr);
     pl->tiles = fmm::M2LTiles();
   } catch (const std::bad_alloc &) {
     return FMM_ERR_MEMORY;
@@ -327,7 +339,8 @@ int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
   if (!(mixed ? fmmdev::m2l_mixed_supported(p->P) : fmmdev::m2l_mfma_supported(p->P))) return FMM_OK;
   try {
     if (pl->groups_kind != p->precision) {
-      pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, mixed);
+      pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, mixed,
+                                         pl->images > 0 ? pl->cycle : nullptr);
       pl->groups_kind = p->precision;
       pl->tiles = fmm::M2LTiles();
     }
@@ -347,7 +360,7 @@ int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans) {
   std::vector<fmm::PlanPart> parts;
   int64_t n = 0;
   for (int i = 0; i < k; i++) {
-    if (!plans[i]) return FMM_ERR_ARG;
+    if (!plans[i] || plans[i]->images != 0) return FMM_ERR_ARG;  // forests of free-space systems only
     n += plans[i]->n;
     parts.push_back({&plans[i]->tree, &plans[i]->lists, &plans[i]->groups, &plans[i]->tiles});
   }
@@ -433,7 +446,7 @@ int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q) {
   if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
   fmm_plan pl;
   try {
-    plan_fill(&pl, n, x, c->p.ncrit, c->p.theta, c->p.nspawn);
+    plan_fill(&pl, n, x, c->p.ncrit, c->p.theta, c->p.nspawn, c->p.images, c->p.cycle);
   } catch (const std::bad_alloc &) {
     FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while building the tree");
   }
@@ -444,6 +457,9 @@ int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const d
   if (!c || !pl) return FMM_ERR_ARG;
   if (pl->n > 0 && (!x || !q)) FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies_plan: NULL array");
   if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  if (pl->images != c->p.images ||
+      (pl->images > 0 && (pl->cycle[0] != c->p.cycle[0] || pl->cycle[1] != c->p.cycle[1] || pl->cycle[2] != c->p.cycle[2])))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies_plan: the plan's periodic images differ from the context's");
   fmm_plan copy = *pl;
   return upload_plan(c, std::move(copy), x, q);
 }
@@ -577,7 +593,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
       c->groups = std::move(pl.groups);
       if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) tiles = std::move(pl.tiles);
     } else {
-      c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed);
+      c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed,
+                                        c->p.images > 0 ? c->p.cycle : nullptr);
     }
     const fmm::M2LGroups &G = c->groups;
     UP(g_dx, G.dx) UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
@@ -603,11 +620,65 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
   if (c->mixed && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
+  c->cyc = c->p.images > 0 ? make_double3(c->p.cycle[0], c->p.cycle[1], c->p.cycle[2]) : make_double3(0, 0, 0);
+  c->far = c->p.images >= 2 && nc > 0;
+  if (c->far) {  // traversePeriodic (include/traversal.h:233-287) as one operator on the root
+    const int P = c->p.P, PP = P * P;
+    const int prange = (int)(0.5 / c->p.theta + 1), nb = 2 * prange + 1;
+    const size_t mb = sizeof(double) * PP * PP;
+    if ((rc = ensure(c, c->per_F, mb)) || (rc = ensure(c, c->per_T, mb)) || (rc = ensure(c, c->per_S, mb)) ||
+        (rc = ensure(c, c->per_cur, mb)) || (rc = ensure(c, c->per_tmp, mb)))
+      return rc;
+    const int nmax = (nb * nb * nb - 1) * nb * nb * nb;
+    if ((rc = ensure(c, c->per_dx, sizeof(double) * 3 * nmax)) || (rc = ensure(c, c->per_kind, sizeof(int) * nmax)))
+      return rc;
+    HIPCHK(hipMemsetAsync(c->per_F.p, 0, mb, c->st));
+    HIPCHK(hipMemsetAsync(c->per_cur.p, 0, mb, c->st));
+    HIPCHK(fmmdev::launch_add_identity(c->per_cur.as<double>(), PP, c->st));
+    HIPCHK(hipMemsetAsync(c->per_kind.p, 0, sizeof(int) * nmax, c->st));  // kind 0: M2M
+    double cyc[3] = {c->p.cycle[0], c->p.cycle[1], c->p.cycle[2]};
+    std::vector<double> dl, ds;
+    for (int lev = 0; lev < c->p.images - 1; lev++) {
+      dl.clear();
+      ds.clear();
+      for (int ix = -prange; ix <= prange; ix++)
+        for (int iy = -prange; iy <= prange; iy++)
+          for (int iz = -prange; iz <= prange; iz++) {
+            if (ix == 0 && iy == 0 && iz == 0) continue;
+            // M2L(root, supercell) with Xperiodic = (i*nb + c) * cycle: dX = -Xperiodic
+            for (int cx = -prange; cx <= prange; cx++)
+              for (int cy = -prange; cy <= prange; cy++)
+                for (int cz = -prange; cz <= prange; cz++)
+                  dl.insert(dl.end(), {-(ix * nb + cx) * cyc[0], -(iy * nb + cy) * cyc[1], -(iz * nb + cz) * cyc[2]});
+            // supercell copy at X + i * cycle; M2M dX = X_parent - X_child
+            ds.insert(ds.end(), {-ix * cyc[0], -iy * cyc[1], -iz * cyc[2]});
+          }
+      const int nl = (int)(dl.size() / 3), ns = (int)(ds.size() / 3);
+      // T_l, then F += T_l cur
+      HIPCHK(hipMemcpyAsync(c->per_dx.p, dl.data(), sizeof(double) * dl.size(), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemsetAsync(c->per_T.p, 0, mb, c->st));
+      HIPCHK(fmmdev::launch_m2l_op_accum(P, c->per_dx.as<double>(), nl, c->per_T.as<double>(), c->st));
+      HIPCHK(fmmdev::launch_matmul(c->per_T.as<double>(), c->per_cur.as<double>(), c->per_F.as<double>(), PP, 1,
+                                   c->st));
+      // S_l = I + sum of the supercell M2M operators, cur = S_l cur
+      HIPCHK(hipStreamSynchronize(c->st));  // dl is reused below
+      HIPCHK(hipMemcpyAsync(c->per_dx.p, ds.data(), sizeof(double) * ds.size(), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemsetAsync(c->per_S.p, 0, mb, c->st));
+      HIPCHK(fmmdev::launch_xlate_ops(P, c->per_dx.as<double>(), c->per_kind.as<int>(), ns, c->per_S.as<double>(), 1,
+                                      c->st));
+      HIPCHK(fmmdev::launch_add_identity(c->per_S.as<double>(), PP, c->st));
+      HIPCHK(fmmdev::launch_matmul(c->per_S.as<double>(), c->per_cur.as<double>(), c->per_tmp.as<double>(), PP, 0,
+                                   c->st));
+      std::swap(c->per_cur, c->per_tmp);
+      HIPCHK(hipStreamSynchronize(c->st));  // ds dies at the next iteration
+      for (int d = 0; d < 3; d++) cyc[d] *= nb;
+    }
+  }
   if (c->xl) {  // the per-level M2M / L2L operators, once per plan
     const int nop = 16 * (t.depth + 1);
     if ((rc = ensure(c, c->xl_ops, sizeof(double) * fmmdev::xlate_op_stride(c->p.P) * nop))) return rc;
     HIPCHK(fmmdev::launch_xlate_ops(c->p.P, c->xl_dx.as<double>(), c->xl_kinds.as<int>(), nop,
-                                    c->xl_ops.as<double>(), c->st));
+                                    c->xl_ops.as<double>(), 0, c->st));
   }
   if (c->use_tiles) {  // harmonics of every group offset, once per plan
     const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * c->groups.ngroup;
@@ -641,7 +712,8 @@ namespace {
 // caller-built tree -> plan (lists built here); returns "" or the reason it was refused
 std::string plan_from_tree(fmm_plan &pl, int n, const double *x, int ncell, const int *iparent,
                            const int *ichild, const int *nchild, const int *ibody, const int *nbody,
-                           const double *X, const double *R, double theta, int nspawn) {
+                           const double *X, const double *R, double theta, int nspawn, int images = 0,
+                           const double *cycle = nullptr) {
   if (n < 0 || ncell < 0 || (n > 0 && !x) ||
       (ncell > 0 && (!iparent || !ichild || !nchild || !ibody || !nbody || !X || !R)))
     return "bad sizes or NULL array";
@@ -652,7 +724,10 @@ std::string plan_from_tree(fmm_plan &pl, int n, const double *x, int ncell, cons
   if (!why.empty()) return why;
   const double t1 = now_ms();
   const double xper[3] = {0, 0, 0};
-  pl.lists = fmm::build_lists(pl.tree, theta, nspawn, xper);
+  pl.images = images;
+  if (images > 0)
+    for (int d = 0; d < 3; d++) pl.cycle[d] = cycle[d];
+  pl.lists = fmm::build_lists(pl.tree, theta, nspawn, xper, images, pl.cycle);
   pl.t_tree = t1 - t0;
   pl.t_lists = now_ms() - t1;
   fmm::get_bounds(n, x, pl.xmin, pl.xmax);
@@ -698,7 +773,7 @@ int fmm_set_tree(fmm_ctx *c, int n, const double *x, const double *q, int ncell,
   fmm_plan pl;
   try {
     const std::string why = plan_from_tree(pl, n, x, ncell, iparent, ichild, nchild, ibody, nbody, X, R,
-                                           c->p.theta, c->p.nspawn);
+                                           c->p.theta, c->p.nspawn, c->p.images, c->p.cycle);
     if (!why.empty()) FMM_FAIL(FMM_ERR_ARG, "fmm_set_tree: " + why);
   } catch (const std::bad_alloc &) {
     FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while adopting the tree");
@@ -747,7 +822,7 @@ int fmm_p2p(fmm_ctx *c) {
     HIPCHK(hipMemsetAsync(c->ay.p, 0, bytes, c->st));
     HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, c->st));
   }
-  const double3 xper = make_double3(0.0, 0.0, 0.0);
+  const double3 xper = c->cyc;  // image shifts of the source codes
   if (c->timing) HIPCHK(hipEventRecord(c->kev[2], c->st));
   HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
       dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
@@ -761,7 +836,7 @@ int fmm_m2l(fmm_ctx *c) {
   if (rc) return rc;
   if (c->tree.ncell == 0) return FMM_OK;
   HIPCHK(hipMemsetAsync(c->L.p, 0, sizeof(double2) * nterm(c) * c->tree.ncell, c->st));
-  const double3 xper = make_double3(0.0, 0.0, 0.0);
+  const double3 xper = c->cyc;  // image shifts of the source codes
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   if (c->use_mfma) {
     if (c->mixed)
@@ -793,6 +868,8 @@ int fmm_m2l(fmm_ctx *c) {
                               c->m2l_src.as<int>(), xper, c->M.as<double2>(), c->L.as<double2>(), 0,
                               c->st));
   }
+  if (c->far)
+    HIPCHK(fmmdev::launch_periodic_far(c->p.P, c->per_F.as<double>(), c->M.as<double2>(), c->L.as<double2>(), c->st));
   if (c->timing) HIPCHK(hipEventRecord(c->kev[1], c->st));
   return FMM_OK;
 }
@@ -876,8 +953,15 @@ int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg) {
   if ((rc = upload(c, tb, h.data(), h.size()))) return done(rc);
   if ((rc = ensure(c, ob, sizeof(double) * 4 * nt))) return done(rc);
   const double *tx = tb.as<double>();
+  // Traversal::direct (include/traversal.h:355-377): images sum the sources
+  // over the cube of shifts |i| <= sum_(l < images) neighbor^l
+  int prange = 0;
+  if (c->p.images > 0) {
+    const int neighbor = (int)(2 * (.5 / c->p.theta + 1) + 1);
+    for (int l = 0, pw = 1; l < c->p.images; l++, pw *= neighbor) prange += pw;
+  }
   hipError_t e = fmmdev::launch_direct(dev_bodies(c), c->n, tx, tx + nt, tx + 2 * (size_t)nt, nt,
-                                       ob.as<double>(), c->p.eps2, c->st);
+                                       ob.as<double>(), c->p.eps2, prange, c->cyc, c->st);
   if (e != hipSuccess) {
     c->err = std::string("direct kernel launch failed: ") + hipGetErrorString(e);
     return done(FMM_ERR_DEVICE);

@@ -69,7 +69,15 @@ hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 
 // with dX = X_child - X_parent; xlate_op_stride(P) doubles each)
 bool xlate_mfma_supported(int P);
 int xlate_op_stride(int P);
-hipError_t launch_xlate_ops(int P, const double *dxs, const int *kinds, int nop, double *ops, hipStream_t s);
+// accum != 0: sum of all operators into ops (row stride P^2, no padding)
+hipError_t launch_xlate_ops(int P, const double *dxs, const int *kinds, int nop, double *ops, int accum,
+                            hipStream_t s);
+// periodic far images: T += sum of the M2L operators of the n offsets dxs
+// (P^2 x P^2, row-major), dense products, and L(root) += F v(M(root))
+hipError_t launch_m2l_op_accum(int P, const double *dxs, int n, double *T, hipStream_t s);
+hipError_t launch_matmul(const double *A, const double *B, double *C, int n, int acc, hipStream_t s);
+hipError_t launch_add_identity(double *A, int n, hipStream_t s);
+hipError_t launch_periodic_far(int P, const double *F, const double2 *M, double2 *L, hipStream_t s);
 // parents of one level in tiles of 16; child8[i*8 + octant] (-1: none)
 hipError_t launch_m2m_mfma(int P, const int *parents, const int *child8, int np, const double *ops8, double2 *M,
                            hipStream_t s);
@@ -89,7 +97,7 @@ hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, 
 
 // Direct sum (include/traversal.h:355-377) of all n sources at nt targets
 // (SoA txyz); writes tpot/tax/tay/taz.
-hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz,
-                         int nt, double *out, double eps2, hipStream_t s);
+hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,
+                         double *out, double eps2, int prange, double3 cyc, hipStream_t s);
 
 }  // namespace fmmdev

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -259,6 +260,7 @@ struct Walker {
   // tasks (in visiting order) instead of being traversed
   int cut = 0;
   std::vector<std::pair<int, int>> *tasks = nullptr;
+  int tag = 0;  // image bits of the source codes this walker emits
 
   void dual(int Ci, int Cj) {
     // dualTreeTraversal (include/traversal.h:117-177)
@@ -273,14 +275,14 @@ struct Walker {
     }
     if (accept) {
       m2l_i.push_back(Ci);
-      m2l_j.push_back(Cj);
+      m2l_j.push_back(Cj | tag);
     } else if (t.nchild[Ci] == 0 && t.nchild[Cj] == 0) {
       if (t.nbody[Cj] == 0) {
         m2l_i.push_back(Ci);
-        m2l_j.push_back(Cj);
+        m2l_j.push_back(Cj | tag);
       } else {
         p2p_i.push_back(Ci);
-        p2p_j.push_back(Cj);
+        p2p_j.push_back(Cj | tag);
       }
     } else {
       split(Ci, Cj);
@@ -390,22 +392,29 @@ int host_threads() {
   return std::max(1, std::min(16, hw));
 }
 
-Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]) {
+Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3], int images, const double *cycle) {
   Lists L;
   L.m2l_ptr.assign(t.ncell + 1, 0);
   L.p2p_ptr.assign(t.ncell + 1, 0);
   if (t.ncell == 0) return L;
-  double xp[3] = {0, 0, 0};
-  if (xper) std::memcpy(xp, xper, sizeof xp);
   const int nth = t.n >= 50000 ? host_threads() : 1;
-  std::vector<Walker> ws;
+  std::deque<Walker> ws;  // segments in emission order (stable addresses)
+  const int nimg = images > 0 ? 27 : 1;
+  for (int img = 0; img < nimg; img++) {
+  double xp[3] = {0, 0, 0};
+  if (images > 0)
+    image_shift(img, cycle, xp);
+  else if (xper)
+    std::memcpy(xp, xper, sizeof xp);
+  const int tag = images > 0 ? img << kImageShift : 0;
   if (nth == 1) {
     ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
-    Walker &w = ws[0];
-    w.m2l_i.reserve(64 * (size_t)t.ncell);
-    w.m2l_j.reserve(64 * (size_t)t.ncell);
-    w.p2p_i.reserve(32 * (size_t)t.leaves.size());
-    w.p2p_j.reserve(32 * (size_t)t.leaves.size());
+    Walker &w = ws.back();
+    w.tag = tag;
+    w.m2l_i.reserve(64 * (size_t)t.ncell / nimg);
+    w.m2l_j.reserve(64 * (size_t)t.ncell / nimg);
+    w.p2p_i.reserve(32 * (size_t)t.leaves.size() / nimg);
+    w.p2p_j.reserve(32 * (size_t)t.leaves.size() / nimg);
     w.dual(0, 0);
   } else {
     // The recursion down to cell pairs of at most `cut` bodies runs here and
@@ -418,18 +427,23 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3])
     top.tasks = &tasks;
     top.dual(0, 0);
     const size_t nblk = std::min<size_t>(tasks.size(), (size_t)nth * 16);
-    for (size_t b = 0; b < nblk; b++) ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
+    const size_t first = ws.size();
+    for (size_t b = 0; b < nblk; b++) {
+      ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
+      ws.back().tag = tag;
+    }
     std::atomic<size_t> next{0};
     auto work = [&] {
       for (size_t b; (b = next.fetch_add(1)) < nblk;) {
         const size_t lo = tasks.size() * b / nblk, hi = tasks.size() * (b + 1) / nblk;
-        for (size_t k = lo; k < hi; k++) ws[b].dual(tasks[k].first, tasks[k].second);
+        for (size_t k = lo; k < hi; k++) ws[first + b].dual(tasks[k].first, tasks[k].second);
       }
     };
     std::vector<std::thread> th;
     for (int i = 1; i < nth; i++) th.emplace_back(work);
     work();
     for (auto &x : th) x.join();
+  }
   }
   std::vector<Seg> sm, sp;
   for (const Walker &w : ws) {
@@ -438,7 +452,7 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3])
     sm.push_back({&w.m2l_i, &w.m2l_j});
     sp.push_back({&w.p2p_i, &w.p2p_j});
     for (size_t k = 0; k < w.p2p_i.size(); k++)
-      L.p2p_body_pairs += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[w.p2p_j[k]];
+      L.p2p_body_pairs += (int64_t)t.nbody[w.p2p_i[k]] * t.nbody[cell_of(w.p2p_j[k])];
   }
   to_csr(t.ncell, sm, L.m2l_ptr, L.m2l_src, L.m2l_targets, nth);
   to_csr(t.ncell, sp, L.p2p_ptr, L.p2p_src, L.p2p_targets, nth);
@@ -540,7 +554,7 @@ struct KeyIndex2 {
 
 }  // namespace
 
-M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level) {
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level, const double *cycle) {
   M2LGroups G;
   const int nc = t.ncell;
   G.rest_ptr.assign(nc + 1, 0);
@@ -553,6 +567,7 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   std::vector<int> gid(np);
   std::vector<int64_t> kvec;   // 3 per group
   std::vector<int> gexp;       // e per group
+  std::vector<int> gimg;       // image per group
   std::vector<double> sc;      // by_level: (R_s, R_t) per group
   KeyIndex2 index;
   const int64_t lim = 1 << 18;
@@ -564,7 +579,7 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     const double xc = t.X[3 * (size_t)c], yc = t.X[3 * (size_t)c + 1], zc = t.X[3 * (size_t)c + 2];
     const int lt = t.level[c];
     for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
-      const int scell = L.m2l_src[p];
+      const int scell = cell_of(L.m2l_src[p]), img = image_of(L.m2l_src[p]);
       const int ls = t.level[scell];
       int e = std::max(lt, ls);
       int64_t k[3];
@@ -595,13 +610,14 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
       }
       const uint64_t key = ((uint64_t)e << 57) | ((uint64_t)(k[0] + lim) << 38) |
                            ((uint64_t)(k[1] + lim) << 19) | (uint64_t)(k[2] + lim);
-      // by_level: the source and target levels are part of the group
-      const uint64_t key2 = by_level ? (((uint64_t)lt << 8) | (uint64_t)ls) + 1 : 0;
+      // the image, and (by_level) the source and target levels, are part of the group
+      const uint64_t key2 = ((uint64_t)img << 20) | (by_level ? (((uint64_t)lt << 8) | (uint64_t)ls) + 1 : 0);
       bool fresh;
       const int g = index.get(key, key2, fresh);
       if (fresh) {
         kvec.insert(kvec.end(), {k[0], k[1], k[2]});
         gexp.push_back(e);
+        gimg.push_back(img);
         if (by_level) sc.insert(sc.end(), {t.R[scell], t.R[c]});
       }
       if (by_level && (t.R[scell] != sc[2 * (size_t)g] || t.R[c] != sc[2 * (size_t)g + 1])) {
@@ -615,7 +631,9 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   G.dx.resize(3 * (size_t)ng);
   for (int g = 0; g < ng; g++) {
     const double u = R0 / std::ldexp(1.0, gexp[g]);
-    for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u;
+    double sh[3] = {0, 0, 0};
+    if (cycle) image_shift(gimg[g], cycle, sh);
+    for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u - sh[d];
   }
   if (by_level) G.scale = std::move(sc);
   G.ngroup = ng;

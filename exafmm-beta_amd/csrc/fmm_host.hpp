@@ -57,9 +57,26 @@ std::string adopt_tree(Tree &t, int n, int ncell, const int *iparent, const int 
                        const int *nchild, const int *ibody, const int *nbody, const double *X,
                        const double *R);
 
-// Traversal::traverse with images = 0 (include/traversal.h:117-177, 90-114,
-// 180-230, 326-352), recording pairs instead of evaluating them.
-Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3]);
+// Source entries of the lists are codes: the source cell in the low 26 bits
+// and, for periodic runs, the image in the bits above (image i = 0..26 is the
+// shift (i/9 - 1, i/3 % 3 - 1, i % 3 - 1) * cycle; 0 without images).
+constexpr int kImageShift = 26;
+constexpr int kCellMask = (1 << kImageShift) - 1;
+inline int image_of(int code) { return code >> kImageShift; }
+inline int cell_of(int code) { return code & kCellMask; }
+inline void image_shift(int img, const double cycle[3], double out[3]) {
+  out[0] = (img / 9 - 1) * cycle[0];
+  out[1] = (img / 3 % 3 - 1) * cycle[1];
+  out[2] = (img % 3 - 1) * cycle[2];
+}
+
+// Traversal::traverse (include/traversal.h:117-177, 90-114, 180-230,
+// 326-352), recording pairs instead of evaluating them.  images > 0: the
+// dual traversal of the root with itself for each of the 27 shifts in the
+// reference's loop order, pairs tagged with their image (the hierarchical
+// far images of traversePeriodic are not pairs; see the periodic operator).
+Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3], int images = 0,
+                  const double *cycle = nullptr);
 
 // M2L pairs grouped by their exact offset X_t - X_s.  Offsets between cells of
 // the reference's octree are integer multiples of R_root / 2^e; a group key is
@@ -81,7 +98,8 @@ struct M2LGroups {
 // by_level: a group also fixes the source and target levels (the scaled FP32
 // operator depends on R_s and R_t); pairs whose radii differ from their
 // group's are left ungrouped.
-M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level = false);
+M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level = false,
+                           const double *cycle = nullptr);
 // Target-stationary layout of the grouped M2L pairs: targets are sorted by
 // (level, octant within the parent, cell) -- cells of one octant class share
 // their offsets -- and cut into tiles of TT targets.  A tile's pairs are

@@ -267,6 +267,7 @@ int fmm_server_create(fmm_server **out, const fmm_params *p, const fmm_server_co
   if (c.slots < 1 || c.slots > 16 || c.batch_bodies < 1 || c.batch_requests < 1 || c.device_budget < 1 ||
       c.host_threads < 0 || !(c.batch_delay_ms >= 0))
     return FMM_ERR_ARG;
+  if (p->images != 0) return FMM_ERR_UNSUPPORTED;  // batches are forests of free-space systems
   // validate the parameters once with a context (reports bad P, theta, ...)
   fmm_ctx *probe = nullptr;
   const int rc = fmm_create(&probe, p);

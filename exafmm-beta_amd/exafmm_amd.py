@@ -69,7 +69,8 @@ class FMMError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("P", ctypes.c_int), ("ncrit", ctypes.c_int), ("nspawn", ctypes.c_int),
                 ("images", ctypes.c_int), ("theta", ctypes.c_double), ("eps2", ctypes.c_double),
-                ("precision", ctypes.c_int), ("device", ctypes.c_int), ("m2l_mode", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("device", ctypes.c_int), ("m2l_mode", ctypes.c_int),
+                ("cycle", ctypes.c_double * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -168,12 +169,17 @@ class FMM:
     """One evaluator context (HIP stream + device arena)."""
 
     def __init__(self, P=10, theta=0.4, ncrit=64, nspawn=5000, eps2=0.0, device=0,
-                 precision=FMM_FP64, m2l_mode=M2L_AUTO):
+                 precision=FMM_FP64, m2l_mode=M2L_AUTO, images=0, cycle=None):
+        """images > 0: periodic with period `cycle` (3 floats, or one for a cube)."""
         L = load()
         p = Params()
         L.fmm_default_params(ctypes.byref(p))
         p.P, p.theta, p.ncrit, p.nspawn, p.eps2 = P, theta, ncrit, nspawn, eps2
-        p.device, p.precision, p.m2l_mode = device, precision, m2l_mode
+        p.device, p.precision, p.m2l_mode, p.images = device, precision, m2l_mode, images
+        if cycle is not None:
+            cyc = np.broadcast_to(np.asarray(cycle, dtype=np.float64), (3,))
+            for d in range(3):
+                p.cycle[d] = cyc[d]
         self.P = P
         self.nterm = P * (P + 1) // 2
         self._h = ctypes.c_void_p()

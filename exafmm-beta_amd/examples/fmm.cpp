@@ -1,8 +1,9 @@
 // Laplace FMM driver with the call sequence of exafmm-beta's examples/fmm.cxx
 // (lines 13-140): same options (Args), same passes, same verification against
 // a direct sum on 100 sampled bodies.  Each pass runs on the GPU through
-// include/exafmm_amd.hpp.  Periodic images, separate source sets (-j), PAPI,
-// DAG output and the timing-regression database are not part of this path.
+// include/exafmm_amd.hpp, periodic images (-i) included.  Separate source sets
+// (-j), PAPI, DAG output and the timing-regression database are not part of
+// this path.
 #include <cmath>
 #include <iostream>
 

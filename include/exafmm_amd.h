@@ -58,12 +58,16 @@ typedef struct {
   int P;          /* expansion order, 2..20 */
   int ncrit;      /* max bodies per leaf (BuildTree(ncrit)) */
   int nspawn;     /* Traversal nspawn: both-sides split threshold */
-  int images;     /* periodic image levels; only 0 is supported */
+  int images;     /* periodic images (Traversal(..., images, ...)): 0 = free space; > 0 = the
+                   * 27 neighbouring copies in the lists plus images - 1 levels of far copies
+                   * (include/traversal.h:233-287, 326-352) */
   double theta;   /* multipole acceptance parameter */
   double eps2;    /* softening added to r^2 in P2P */
   int precision;  /* fmm_precision */
   int device;     /* HIP device ordinal */
   int m2l_mode;   /* fmm_m2l_mode */
+  double cycle[3];  /* period in x, y, z (the `cycle` argument of Traversal::traverse); used
+                     * when images > 0 */
 } fmm_params;
 
 typedef struct {
@@ -177,8 +181,10 @@ int fmm_downward(fmm_ctx *c);
 int fmm_evaluate(fmm_ctx *c);
 int fmm_synchronize(fmm_ctx *c);
 
-/* Traversal::direct with images = 0 (include/traversal.h:355-377): direct
- * sum over all bodies of the context (tree order) at nt target points xt
+/* Traversal::direct (include/traversal.h:355-377): direct sum over all
+ * bodies of the context (tree order) -- with images > 0 over the cube of
+ * shifts (ix, iy, iz) * cycle, |i| <= sum_(l < images) neighbor^l -- at nt
+ * target points xt
  * (nt x 3); trg (nt x 4) is overwritten.  R^2 = 0 pairs are skipped like in
  * Kernel::P2P.  Uses IEEE sqrt and division, independently of the P2P kernel. */
 int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg);

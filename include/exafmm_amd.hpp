@@ -214,6 +214,8 @@ class Kernel {
   fmm_ctx *ctx = nullptr;
   real_t theta = 0.4;
   int nspawn = 5000;
+  int images = 0;
+  vec3 cycle = 2 * M_PI;      // the period the context was made with (images > 0)
   Cells *uploaded = nullptr;  // cells whose tree is on the device
 
   Kernel(int _P, real_t _eps2, complex_t _wavek, int device = 0)
@@ -225,9 +227,11 @@ class Kernel {
   Kernel &operator=(const Kernel &) = delete;
 
   //! (re)create the context with the traversal parameters
-  void configure(real_t _theta, int _nspawn) {
+  void configure(real_t _theta, int _nspawn, int _images, vec3 _cycle) {
     theta = _theta;
     nspawn = _nspawn;
+    images = _images;
+    cycle = _cycle;
     if (ctx) fmm_destroy(ctx);
     ctx = nullptr;
     fmm_params p;
@@ -236,6 +240,8 @@ class Kernel {
     p.eps2 = eps2;
     p.theta = theta;
     p.nspawn = nspawn;
+    p.images = images;
+    for (int d = 0; d < 3; d++) p.cycle[d] = cycle[d];
     p.device = dev;
     const int rc = fmm_create(&ctx, &p);
     if (rc != FMM_OK) {
@@ -250,7 +256,7 @@ class Kernel {
   }
   //! upload the tree held by cells (bodies in tree order, as buildTree leaves them)
   void upload(Cells &cells) {
-    if (!ctx) configure(theta, nspawn);
+    if (!ctx) configure(theta, nspawn, images, cycle);
     const int nc = (int)cells.size();
     const int n = nc ? cells[0].NBODY : 0;
     B_iter B0 = nc ? cells[0].BODY : B_iter();
@@ -321,7 +327,7 @@ class UpDownPass {
   }
 };
 
-//! include/traversal.h:291-377 (images = 0, icells == jcells)
+//! include/traversal.h:291-377 (icells == jcells)
 class Traversal {
   Kernel &kernel;
   const real_t theta;
@@ -331,25 +337,40 @@ class Traversal {
  public:
   Traversal(Kernel &_kernel, real_t _theta, int _nspawn, int _images, const char * = "./")
       : kernel(_kernel), theta(_theta), nspawn(_nspawn), images(_images) {
-    if (images != 0) throw Error("periodic images are not supported (images must be 0)");
-    kernel.configure(theta, nspawn);
+    kernel.configure(theta, nspawn, images, kernel.cycle);
+  }
+  //! images > 0 and a period other than the context's: remake the context
+  //! (the lists depend on the period) and, if a tree was up, re-run its upward pass
+  void use_cycle(vec3 cycle) {
+    if (images == 0 || (cycle[0] == kernel.cycle[0] && cycle[1] == kernel.cycle[1] && cycle[2] == kernel.cycle[2]))
+      return;
+    Cells *up = kernel.uploaded;
+    kernel.configure(theta, nspawn, images, cycle);
+    kernel.uploaded = nullptr;
+    if (up) {
+      kernel.upload(*up);
+      kernel.check(fmm_upward(kernel.ctx), "fmm_upward");
+    }
   }
   void initListCount(Cells &) {}
   void initWeight(Cells &) {}
-  void traverse(Cells &icells, Cells &jcells, vec3, bool, real_t = 1) {
+  void traverse(Cells &icells, Cells &jcells, vec3 cycle, bool, real_t = 1) {
     if (icells.empty() || jcells.empty()) return;
     if (&icells != &jcells) throw Error("traverse: separate target and source trees are not supported");
     logger::startTimer("Traverse");
+    use_cycle(cycle);
     if (kernel.uploaded != &icells) kernel.upload(icells);
     kernel.check(fmm_p2p(kernel.ctx), "fmm_p2p");
     kernel.check(fmm_m2l(kernel.ctx), "fmm_m2l");
     kernel.check(fmm_synchronize(kernel.ctx), "fmm_synchronize");
     logger::stopTimer("Traverse");
   }
-  //! direct sum of jbodies at the positions of ibodies (TRG += ), on the device
-  void direct(Bodies &ibodies, Bodies &jbodies, vec3) {
+  //! direct sum of jbodies (and, with images > 0, of their copies shifted by
+  //! (ix, iy, iz) * cycle) at the positions of ibodies (TRG += ), on the device
+  void direct(Bodies &ibodies, Bodies &jbodies, vec3 cycle) {
     const int nj = (int)jbodies.size(), ni = (int)ibodies.size();
-    if (!kernel.ctx) kernel.configure(theta, nspawn);
+    if (!kernel.ctx) kernel.configure(theta, nspawn, images, images ? cycle : kernel.cycle);
+    use_cycle(cycle);
     std::vector<double> xj(3 * (size_t)nj), qj(nj), xi(3 * (size_t)ni), trg(4 * (size_t)ni);
     for (int j = 0; j < nj; j++) {
       for (int d = 0; d < 3; d++) xj[3 * (size_t)j + d] = jbodies[j].X[d];

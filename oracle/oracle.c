@@ -772,6 +772,80 @@ void orc_fmm(const orc_kernel *k, const orc_tree *t, const double *x, const doub
   if (!Lout) free(L);
 }
 
+/* Traversal::traversePeriodic  include/traversal.h:233-287 (target Ci0 = the
+ * root, source Cj0 = the root of the same tree) */
+static void traverse_periodic(const orc_kernel *k, const orc_tree *t, double theta, int images,
+                              const double *cycle0, const cplx *M, cplx *L) {
+  const int NT = k->NTERM;
+  const int prange = (int)(.5 / theta + 1), neighbor = 2 * prange + 1;
+  cplx *Mci = (cplx *)malloc(sizeof(cplx) * NT), *Mcopy = (cplx *)malloc(sizeof(cplx) * NT);
+  memcpy(Mci, M, sizeof(cplx) * NT); /* *Ci = *Cj0 */
+  const double *X = t->X;            /* Ci->X = Cj0->X = Ci0->X */
+  double cycle[3] = {cycle0[0], cycle0[1], cycle0[2]};
+  for (int level = 0; level < images - 1; level++) {
+    for (int ix = -prange; ix <= prange; ix++)
+      for (int iy = -prange; iy <= prange; iy++)
+        for (int iz = -prange; iz <= prange; iz++) {
+          if (ix == 0 && iy == 0 && iz == 0) continue;
+          for (int cx = -prange; cx <= prange; cx++)
+            for (int cy = -prange; cy <= prange; cy++)
+              for (int cz = -prange; cz <= prange; cz++) {
+                double xper[3] = {(ix * neighbor + cx) * cycle[0], (iy * neighbor + cy) * cycle[1],
+                                  (iz * neighbor + cz) * cycle[2]};
+                orc_m2l(k, X, X, xper, (const double *)Mci, (double *)L);
+              }
+        }
+    /* the image copies (children of Ci) take Ci's multipole before the M2M */
+    memcpy(Mcopy, Mci, sizeof(cplx) * NT);
+    for (int ix = -prange; ix <= prange; ix++)
+      for (int iy = -prange; iy <= prange; iy++)
+        for (int iz = -prange; iz <= prange; iz++) {
+          if (ix == 0 && iy == 0 && iz == 0) continue;
+          double Xc[3] = {X[0] + ix * cycle[0], X[1] + iy * cycle[1], X[2] + iz * cycle[2]};
+          orc_m2m(k, X, Xc, (const double *)Mcopy, (double *)Mci);
+        }
+    for (int d = 0; d < 3; d++) cycle[d] *= neighbor;
+  }
+  free(Mci);
+  free(Mcopy);
+}
+
+void orc_fmm_periodic(const orc_kernel *k, const orc_tree *t, const double *x, const double *q,
+                      double theta, int nspawn, int images, const double *cycle, double *trg) {
+  if (t->ncell == 0) return;
+  size_t sz = sizeof(cplx) * (size_t)t->ncell * k->NTERM;
+  cplx *M = (cplx *)malloc(sz), *L = (cplx *)malloc(sz);
+  orc_upward(k, t, x, q, (double *)M);
+  memset(L, 0, sz);
+  trav_ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = t;
+  c.theta = theta;
+  c.nspawn = nspawn;
+  c.k = k;
+  c.x = x;
+  c.q = q;
+  c.trg = trg;
+  c.M = M;
+  c.L = L;
+  if (images == 0) {
+    dual_tree(&c, 0, 0);
+  } else {
+    for (int ix = -1; ix <= 1; ix++)
+      for (int iy = -1; iy <= 1; iy++)
+        for (int iz = -1; iz <= 1; iz++) {
+          c.xper[0] = ix * cycle[0];
+          c.xper[1] = iy * cycle[1];
+          c.xper[2] = iz * cycle[2];
+          dual_tree(&c, 0, 0);
+        }
+    traverse_periodic(k, t, theta, images, cycle, M, L);
+  }
+  orc_downward(k, t, x, (double *)L, trg);
+  free(M);
+  free(L);
+}
+
 void orc_eval_lists(const orc_kernel *k, const orc_tree *t, const double *x, const double *q,
                     int64_t npair, const int *ci, const int *cj, const int *type, double *trg,
                     double *L, const double *M) {

@@ -80,6 +80,15 @@ void orc_fmm(const orc_kernel *k, const orc_tree *t, const double *x,
              const double *q, double theta, int nspawn, double *trg,
              double *M, double *L);
 
+/* orc_fmm with periodic images (images > 0): Traversal::traverse runs the
+ * dual traversal for the 27 shifts (ix, iy, iz) * cycle, ix..iz in -1..1
+ * (include/traversal.h:326-352), then traversePeriodic (:233-287) adds the
+ * images - 1 levels of supercell copies of the root to the root's local
+ * expansion by M2L and M2M; then downwardPass. */
+void orc_fmm_periodic(const orc_kernel *k, const orc_tree *t, const double *x,
+                      const double *q, double theta, int nspawn, int images,
+                      const double *cycle, double *trg);
+
 /* Same evaluation driven by explicit pair lists (as recorded by
  * orc_traverse), used to time the reference operators on a list sample. */
 void orc_eval_lists(const orc_kernel *k, const orc_tree *t, const double *x,

@@ -74,6 +74,8 @@ def lib():
         L.orc_lists_count.argtypes = [_vp, ctypes.c_int]
         L.orc_lists_get.argtypes = [_vp, _ip, _ip, _ip]
         L.orc_fmm.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_fmm_periodic.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp]
         L.orc_eval_lists.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_int64, _ip, _ip, _ip, _dp,
                                      _dp, _dp]
         L.orc_upward.argtypes = [_vp, _vp, _dp, _dp, _dp]
@@ -246,6 +248,15 @@ class Tree:
                   _ptr(M.view(np.float64)) if want_ml else None,
                   _ptr(Lc.view(np.float64)) if want_ml else None)
         return (trg, M, Lc) if want_ml else trg
+
+    def fmm_periodic(self, kernel, x_sorted, q_sorted, theta, nspawn, images, cycle):
+        """orc_fmm_periodic: 27 image traversals + traversePeriodic (include/traversal.h)."""
+        xs, qs = _c(x_sorted), _c(q_sorted)
+        trg = np.zeros((self.n, 4))
+        cyc = _c(np.broadcast_to(np.asarray(cycle, dtype=np.float64), (3,)))
+        lib().orc_fmm_periodic(kernel.h, self.h, _ptr(xs), _ptr(qs), theta, nspawn, images, _ptr(cyc),
+                               _ptr(trg))
+        return trg
 
     def upward(self, kernel, x_sorted, q_sorted):
         M = np.zeros((self.ncell, kernel.nterm), np.complex128)

@@ -91,6 +91,44 @@ def test_example_driver_prints_reference_errors():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("images", [1, 2])
+def test_example_driver_periodic(images):
+    """examples/fmm.cxx with --images (cycle 2 pi, the example's): the printed
+    FMM-vs-direct errors equal the oracle's for the same system, 100 sampled
+    targets and the same shifts.  At images = 1 both sums cover the 27 boxes;
+    at images >= 2 the reference's FMM (traversePeriodic, include/traversal.h:
+    233-287) and its direct sum (include/traversal.h:355-377) cover different
+    sets of boxes, so the printed error is that difference, not FMM error."""
+    n, P, theta, cyc = 2048, 10, 0.4, 2 * np.pi
+    exe = os.path.join(PKG, "lib", "fmm")
+    out = subprocess.run([exe, "-n", str(n), "-i", str(images), "-v"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = {k: float(v) for k, v in re.findall(r"Rel\. L2 Error \((pot|acc)\)\s*:\s*(\S+)", out.stdout)}
+    assert set(got) == {"pot", "acc"}, out.stdout
+
+    x, q = O.data_cube(n, 0), O.data_source(n, 0)
+    t = O.Tree(x, 64)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(P)
+    fmm = t.fmm_periodic(k, xs, qs, theta, 5000, images, cyc)
+    idx = np.arange(100) * (n // 100)  # Dataset::sampleBodies (include/dataset.h)
+    neighbor = int(2 * (0.5 / theta + 1) + 1)
+    prange = sum(neighbor ** l for l in range(images))
+    d = np.zeros((len(idx), 4))
+    for ix in range(-prange, prange + 1):
+        for iy in range(-prange, prange + 1):
+            for iz in range(-prange, prange + 1):
+                d += O.direct(k, xs[idx], xs + np.array([ix, iy, iz]) * cyc, qs)
+    f = fmm[idx]
+    pot = np.sqrt(np.sum((d[:, 0] - f[:, 0]) ** 2) / np.sum(d[:, 0] ** 2))
+    acc = np.sqrt(np.sum((d[:, 1:] - f[:, 1:]) ** 2) / np.sum(d[:, 1:] ** 2))
+    assert abs(got["pot"] - pot) <= 1e-6 * pot and abs(got["acc"] - acc) <= 1e-6 * acc, (got, pot, acc)
+    if images == 1:
+        assert pot < 1e-4 and acc < 1e-4
+
+
+@pytest.mark.gpu
 def test_set_tree_equals_set_bodies():
     F = _F()
     x = O.data_plummer(6000, 4)

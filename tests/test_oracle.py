@@ -254,3 +254,52 @@ def test_tree_invariants():
         assert np.all(t.iparent[ch] == c)
         assert t.nbody[ch].sum() == t.nbody[c]
         assert np.allclose(t.R[ch], t.R[c] / 2)
+
+
+# ---------------------------------------------------------------- periodic images
+def _image_sum(k, xs, qs, shifts):
+    """Brute force: every target against every source shifted by each of `shifts`."""
+    out = np.zeros((len(xs), 4))
+    for s in shifts:
+        out += O.direct(k, xs, xs + np.asarray(s), qs)
+    return out
+
+
+def test_periodic_one_image_level_is_the_27_box_sum():
+    """images = 1: the dual traversal over the 27 shifts (include/traversal.h:335-347)
+    and no supercells: the FMM of the 3x3x3 copies."""
+    x, q = O.data_cube(300, 4), O.data_source(300, 4)
+    t = O.Tree(x, 32)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(10)
+    cyc = 2 * np.pi
+    trg = t.fmm_periodic(k, xs, qs, 0.4, 5000, 1, cyc)
+    shifts = [(ix * cyc, iy * cyc, iz * cyc) for ix in (-1, 0, 1) for iy in (-1, 0, 1) for iz in (-1, 0, 1)]
+    d = _image_sum(k, xs, qs, shifts)
+    assert _rel(trg[:, 0], d[:, 0]) < 1e-5 and _rel(trg[:, 1:], d[:, 1:]) < 1e-4
+
+
+def test_periodic_supercell_level_adds_its_boxes():
+    """images = 2: traversePeriodic (include/traversal.h:233-287) adds the root's
+    copies at (i*nb + c) * cycle, i != 0, c in [-prange, prange]^3 (prange =
+    int(.5/theta + 1), nb = 2 prange + 1) by M2L; checked against brute force
+    over exactly the near 27 and those boxes."""
+    x, q = O.data_cube(120, 5), O.data_source(120, 5)
+    t = O.Tree(x, 32)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(10)
+    cyc, theta = 2 * np.pi, 0.4
+    pr = int(0.5 / theta + 1)  # 2: the shell at distance 2 belongs to neither part
+    nb = 2 * pr + 1
+    rng = range(-pr, pr + 1)
+    shifts = [(ix * cyc, iy * cyc, iz * cyc) for ix in (-1, 0, 1) for iy in (-1, 0, 1) for iz in (-1, 0, 1)]
+    shifts += [((ix * nb + cx) * cyc, (iy * nb + cy) * cyc, (iz * nb + cz) * cyc)
+               for ix in rng for iy in rng for iz in rng if (ix, iy, iz) != (0, 0, 0)
+               for cx in rng for cy in rng for cz in rng]
+    trg = t.fmm_periodic(k, xs, qs, theta, 5000, 2, cyc)
+    d = _image_sum(k, xs, qs, shifts)
+    assert _rel(trg[:, 0], d[:, 0]) < 1e-5 and _rel(trg[:, 1:], d[:, 1:]) < 1e-4
+    # the supercell part alone, against its boxes
+    near = t.fmm_periodic(k, xs, qs, theta, 5000, 1, cyc)
+    far = _image_sum(k, xs, qs, shifts[27:])
+    assert _rel((trg - near)[:, 0], far[:, 0]) < 3e-5 and _rel((trg - near)[:, 1:], far[:, 1:]) < 3e-5
