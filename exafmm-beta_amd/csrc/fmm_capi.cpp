@@ -9,6 +9,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fmm_device.hpp"
@@ -18,9 +19,26 @@ namespace {
 
 // Grow-only device buffer: the arena stays resident across calls so repeated
 // solves (successive MD steps, served requests) do not reallocate.
+// Owns its allocation: freed with the context (the owner makes its device
+// current first); moves transfer it, copies are not allowed.
 struct DBuf {
   void *p = nullptr;
   size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    return *this;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
@@ -72,6 +90,7 @@ struct fmm_ctx {
   // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
   bool mixed = false;
   DBuf g_sc, cr, M32;
+  DBuf dip;  // dipole partial sums
 };
 
 struct fmm_plan {
@@ -238,19 +257,12 @@ void fmm_destroy(fmm_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->p.device);
   if (c->st) (void)hipStreamSynchronize(c->st);
-  DBuf *bufs[] = {&c->x,      &c->y,      &c->z,           &c->q,         &c->pot,     &c->ax,
-                  &c->ay,     &c->az,     &c->cx,          &c->cy,        &c->cz,      &c->ibody,
-                  &c->nbody,  &c->ichild, &c->nchild,      &c->iparent,   &c->M,       &c->L,
-                  &c->leaves, &c->internal, &c->level_cells, &c->body_leaf, &c->m2l_ptr, &c->m2l_src,
-                  &c->m2l_tgt, &c->p2p_ptr, &c->p2p_src,     &c->p2p_tgt,   &c->g_tgt,
-                  &c->g_src,  &c->g_dx,   &c->g_chunk,     &c->r_ptr,     &c->r_src,   &c->r_tgt};
-  for (DBuf *b : bufs) release(*b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
     if (e) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
-  delete c;
+  delete c;  // the device buffers are freed here, with the device current
 }
 
 const char *fmm_last_error(const fmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -976,6 +988,44 @@ int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg) {
   for (int i = 0; i < nt; i++)
     for (int k = 0; k < 4; k++) trg[4 * (size_t)i + k] = o[(size_t)k * nt + i];
   return done(FMM_OK);
+}
+
+int fmm_dipole_correction(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->n == 0) return FMM_OK;
+  const double *cy = c->p.cycle;
+  if (!(cy[0] > 0 && cy[1] > 0 && cy[2] > 0)) FMM_FAIL(FMM_ERR_ARG, "fmm_dipole_correction: cycle must be > 0");
+  if ((rc = ensure(c, c->dip, sizeof(double) * 3 * fmmdev::dipole_partials(c->n)))) return rc;
+  const double coef = 4 * M_PI / (3 * cy[0] * cy[1] * cy[2]);
+  HIPCHK(fmmdev::launch_dipole_correction(dev_bodies(c), c->n, c->dip.as<double>(), coef, c->st));
+  return FMM_OK;
+}
+
+int fmm_md_evaluate(fmm_ctx *c, int n, const double *x, const double *q, double *p, double *f) {
+  if (!c) return FMM_ERR_ARG;
+  if (n < 0 || (n > 0 && (!x || !q || !p || !f))) FMM_FAIL(FMM_ERR_ARG, "fmm_md_evaluate: bad n or NULL array");
+  const double *cy = c->p.cycle;
+  if (!(cy[0] > 0 && cy[1] > 0 && cy[2] > 0)) FMM_FAIL(FMM_ERR_ARG, "fmm_md_evaluate: cycle must be > 0");
+  // wrap into [-cycle/2, cycle/2] (vec.h:217-230), on a copy: the caller's x is left as it is
+  std::vector<double> xw(x, x + 3 * (size_t)n);
+  for (size_t i = 0; i < (size_t)n; i++)
+    for (int d = 0; d < 3; d++) {
+      double &v = xw[3 * i + d];
+      if (v < -cy[d] / 2) v += cy[d];
+      if (v > cy[d] / 2) v -= cy[d];
+    }
+  int rc = fmm_set_bodies(c, n, xw.data(), q);
+  if (rc || n == 0) return rc;
+  if ((rc = fmm_evaluate(c))) return rc;
+  if ((rc = fmm_dipole_correction(c))) return rc;
+  std::vector<double> trg(4 * (size_t)n);
+  if ((rc = fmm_get_targets(c, trg.data(), 1))) return rc;
+  for (size_t i = 0; i < (size_t)n; i++) {  // TRG starts from the caller's p, f (md_distributed.cxx:209-212)
+    p[i] += trg[4 * i];
+    for (int d = 0; d < 3; d++) f[3 * i + d] += trg[4 * i + 1 + d];
+  }
+  return FMM_OK;
 }
 
 int fmm_synchronize(fmm_ctx *c) {

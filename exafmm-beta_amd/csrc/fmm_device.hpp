@@ -97,6 +97,9 @@ hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, 
 
 // Direct sum (include/traversal.h:355-377) of all n sources at nt targets
 // (SoA txyz); writes tpot/tax/tay/taz.
+// getDipole + dipoleCorrection on the bodies' targets; part: 3 * dipole_partials(n) doubles
+int dipole_partials(int n);
+hipError_t launch_dipole_correction(DevBodies B, int n, double *part, double coef, hipStream_t s);
 hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,
                          double *out, double eps2, int prange, double3 cyc, hipStream_t s);
 

@@ -28,6 +28,7 @@ SYMBOLS = [
     "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
     "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
+    "fmm_dipole_correction", "fmm_md_evaluate",
 ]
 
 FMM_FP64 = 0
@@ -114,7 +115,7 @@ def load():
         L.fmm_set_bodies.argtypes = [vp, ctypes.c_int, _dp, _dp]
         L.fmm_set_charges.argtypes = [vp, _dp]
         for f in ("fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
-                  "fmm_synchronize"):
+                  "fmm_synchronize", "fmm_dipole_correction"):
             getattr(L, f).argtypes = [vp]
         L.fmm_get_targets.argtypes = [vp, _dp, ctypes.c_int]
         L.fmm_get_expansions.argtypes = [vp, _dp, _dp]
@@ -130,6 +131,7 @@ def load():
         L.fmm_plan_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
         L.fmm_make_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         L.fmm_direct.argtypes = [vp, ctypes.c_int, _dp, _dp]
+        L.fmm_md_evaluate.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.fmm_set_tree.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
                                    _dp, _dp]
         L.fmm_plan_from_tree.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
@@ -276,6 +278,25 @@ class FMM:
         trg = np.zeros((len(xt), 4))
         self._check(load().fmm_direct(self._h, len(xt), _ptr(xt), _ptr(trg)), "fmm_direct")
         return trg
+
+    def dipole_correction(self):
+        """UpDownPass::getDipole + dipoleCorrection on the device targets."""
+        self._check(load().fmm_dipole_correction(self._h), "fmm_dipole_correction")
+
+    def md_evaluate(self, x, q, p=None, f=None):
+        """FMM_FMM (wrappers/md_distributed.cxx:190-264) on flat arrays: returns
+        (p, f) with the periodic potential and acceleration added to p, f."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        n = len(q)
+        if len(x) != n:
+            raise ValueError("x and q lengths differ")
+        p = np.zeros(n) if p is None else np.array(p, dtype=np.float64)
+        f = np.zeros((n, 3)) if f is None else np.array(f, dtype=np.float64).reshape(n, 3)
+        self._check(load().fmm_md_evaluate(self._h, n, _ptr(x), _ptr(q), _ptr(p), _ptr(f)),
+                    "fmm_md_evaluate")
+        self.n = n
+        return p, f
 
     def stats(self):
         s = Stats()

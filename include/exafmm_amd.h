@@ -181,6 +181,21 @@ int fmm_downward(fmm_ctx *c);
 int fmm_evaluate(fmm_ctx *c);
 int fmm_synchronize(fmm_ctx *c);
 
+/* UpDownPass::getDipole(bodies, X0 = 0) + dipoleCorrection(bodies, dipole, n,
+ * cycle) (include/up_down_pass.h:77-99) on the context's targets, on the
+ * device: D = sum q x over the bodies as set; pot -= 4 pi |D|^2 / (3 V n q),
+ * acc -= 4 pi D / (3 V), V = cycle[0] cycle[1] cycle[2].  As in the reference
+ * a body with q = 0 gets a non-finite potential. */
+int fmm_dipole_correction(fmm_ctx *c);
+
+/* FMM_FMM (wrappers/md_distributed.cxx:190-264), one process: positions
+ * wrapped into [-cycle/2, cycle/2] (vec.h:217-230; x itself is not changed),
+ * tree, lists and the periodic FMM of the context's parameters, the dipole
+ * correction, then p[i] += potential and f[3i..3i+2] += acceleration in the
+ * caller's order.  The tree's bounds are those of the wrapped bodies of this
+ * call (the reference keeps the bounds of its last FMM_Partition). */
+int fmm_md_evaluate(fmm_ctx *c, int n, const double *x, const double *q, double *p, double *f);
+
 /* Traversal::direct (include/traversal.h:355-377): direct sum over all
  * bodies of the context (tree order) -- with images > 0 over the cube of
  * shifts (ix, iy, iz) * cycle, |i| <= sum_(l < images) neighbor^l -- at nt

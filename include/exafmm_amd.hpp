@@ -325,6 +325,22 @@ class UpDownPass {
     }
     logger::stopTimer("Downward pass");
   }
+  //! include/up_down_pass.h:77-86: sum over bodies of (X - X0) q
+  vec3 getDipole(Bodies &bodies, vec3 X0) {
+    vec3 dipole = 0.0;
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++)
+      for (int d = 0; d < 3; d++) dipole[d] += (B->X[d] - X0[d]) * B->SRC;
+    return dipole;
+  }
+  //! include/up_down_pass.h:89-99: TRG -= 4 pi / (3 V) (|D|^2 / (N q), D)
+  void dipoleCorrection(Bodies &bodies, vec3 dipole, int numBodies, vec3 cycle) {
+    const real_t coef = 4 * M_PI / (3 * cycle[0] * cycle[1] * cycle[2]);
+    const real_t dd = dipole[0] * dipole[0] + dipole[1] * dipole[1] + dipole[2] * dipole[2];
+    for (B_iter B = bodies.begin(); B != bodies.end(); B++) {
+      B->TRG[0] -= coef * dd / numBodies / B->SRC;
+      for (int d = 0; d < 3; d++) B->TRG[d + 1] -= coef * dipole[d];
+    }
+  }
 };
 
 //! include/traversal.h:291-377 (icells == jcells)

@@ -931,3 +931,109 @@ int orc_sample_indices(int n, int ntarget, int *idx) {
   for (int i = 0; i < n; i++) idx[i] = i;
   return n;
 }
+
+/* ---------------------------------------------------------------- periodic MD pieces */
+
+/* vec.h:217-230: wrap one coordinate triple into [-cycle/2, cycle/2]; returns
+ * the bit mask of the axes moved (unwrap, vec.h:231-235, reverses it). */
+int orc_wrap(double *x, const double *cycle) {
+  int iw = 0;
+  for (int d = 0; d < 3; d++) {
+    if (x[d] < -cycle[d] / 2) {
+      x[d] += cycle[d];
+      iw |= 1 << d;
+    }
+    if (x[d] > cycle[d] / 2) {
+      x[d] -= cycle[d];
+      iw |= 1 << d;
+    }
+  }
+  return iw;
+}
+
+/* UpDownPass::getDipole(bodies, X0 = 0) and dipoleCorrection,
+ * include/up_down_pass.h:77-99: trg -= 4 pi / (3 V) * (|D|^2 / (N q_i), D). */
+void orc_dipole_correction(int n, const double *x, const double *q, const double *cycle, double *trg) {
+  double D[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) D[d] += x[3 * i + d] * q[i];
+  const double coef = 4 * M_PI / (3 * cycle[0] * cycle[1] * cycle[2]);
+  const double dd = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+  for (int i = 0; i < n; i++) {
+    trg[4 * i] -= coef * dd / n / q[i];
+    for (int d = 0; d < 3; d++) trg[4 * i + 1 + d] -= coef * D[d];
+  }
+}
+
+/* Ewald summation, include/ewald.h (tin-foil boundary):
+ *   wave part  (initWaves :64-88, dft :26-40, factor :163-180, idft :43-60):
+ *     k = (l, m, n) over the half space l > 0 | (l = 0, m > 0) | (l = m = 0, n > 0)
+ *     with l^2 + m^2 + n^2 <= ksize^2; S(k) = sum_j q_j e^{i k.x_j 2pi/L};
+ *     phi_i += 2/(sigma V) sum_k e^{-K^2/4 alpha^2}/K^2 Re(S(k) e^{-i K.x_i})
+ *   real part  (P2P :90-110): every pair within the cutoff,
+ *     phi_i += q_j erfc(alpha r)/r, grad_i -= dX q_j (2/sqrt(pi) e^{-a^2 r^2}/(a r)^2
+ *     + erfc(a r)/(a r)^3) a^3
+ *   self term  (:156-160): phi_i -= 2/sqrt(pi) q_i alpha.
+ * The reference finds real-part pairs with a cell search that gives every
+ * (target leaf, source leaf) pair the image that wraps their centres
+ * (Neighbor, :113-137); here every body pair takes its own minimum image,
+ * which is the sum that search is meant to produce (they differ only by
+ * pairs the search places across the box boundary).  trg += the result. */
+void orc_ewald(int n, const double *x, const double *q, int ksize, double alpha, double sigma, double cutoff,
+               const double *cycle, double *trg) {
+  double scale[3];
+  for (int d = 0; d < 3; d++) scale[d] = 2 * M_PI / cycle[d];
+  const double coef = 2 / sigma / cycle[0] / cycle[1] / cycle[2];
+  const double coef2 = 1 / (4 * alpha * alpha);
+  const int kmax = ksize, kmaxsq = ksize * ksize;
+  for (int l = 0; l <= kmax; l++) {
+    const int mmin = l == 0 ? 0 : -kmax;
+    for (int m = mmin; m <= kmax; m++) {
+      const int nmin = (l == 0 && m == 0) ? 1 : -kmax;
+      for (int nn = nmin; nn <= kmax; nn++) {
+        if (l * l + m * m + nn * nn > kmaxsq) continue;
+        const double K[3] = {l, m, nn};
+        double re = 0, im = 0;
+        for (int j = 0; j < n; j++) {
+          double th = 0;
+          for (int d = 0; d < 3; d++) th += K[d] * x[3 * j + d] * scale[d];
+          re += q[j] * cos(th);
+          im += q[j] * sin(th);
+        }
+        double K2 = 0;
+        for (int d = 0; d < 3; d++) K2 += (K[d] * scale[d]) * (K[d] * scale[d]);
+        const double factor = coef * exp(-K2 * coef2) / K2;
+        re *= factor;
+        im *= factor;
+        for (int i = 0; i < n; i++) {
+          double th = 0;
+          for (int d = 0; d < 3; d++) th += K[d] * x[3 * i + d] * scale[d];
+          const double c = cos(th), s = sin(th);
+          const double dtmp = re * s - im * c;
+          trg[4 * i] += re * c + im * s;
+          for (int d = 0; d < 3; d++) trg[4 * i + 1 + d] -= dtmp * K[d] * scale[d];
+        }
+      }
+    }
+  }
+  const double c2 = cutoff * cutoff;
+  for (int i = 0; i < n; i++) {
+    double p = 0, g[3] = {0, 0, 0};
+    for (int j = 0; j < n; j++) {
+      double dX[3];
+      for (int d = 0; d < 3; d++) dX[d] = x[3 * i + d] - x[3 * j + d];
+      orc_wrap(dX, cycle);
+      const double R2 = dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2];
+      if (0 < R2 && R2 < c2) {
+        const double R2s = R2 * alpha * alpha, Rs = sqrt(R2s);
+        const double invRs = 1 / Rs, invR2s = invRs * invRs, invR3s = invR2s * invRs;
+        double dtmp = q[j] * (M_2_SQRTPI * exp(-R2s) * invR2s + erfc(Rs) * invR3s);
+        dtmp *= alpha * alpha * alpha;
+        p += q[j] * erfc(Rs) * invRs * alpha;
+        for (int d = 0; d < 3; d++) g[d] -= dX[d] * dtmp;
+      }
+    }
+    trg[4 * i] += p - M_2_SQRTPI * q[i] * alpha;
+    for (int d = 0; d < 3; d++) trg[4 * i + 1 + d] += g[d];
+  }
+}

@@ -109,6 +109,15 @@ void orc_direct(const orc_kernel *k, int ni, const double *xi, double *trgi,
 void orc_rel_l2(int n, const double *trg_ref, const double *trg, double *pot_rel,
                 double *acc_rel);
 
+/* vec.h:217-230 wrap into [-cycle/2, cycle/2]; returns the moved-axes mask */
+int orc_wrap(double *x, const double *cycle);
+/* UpDownPass::getDipole (X0 = 0) + dipoleCorrection, include/up_down_pass.h:77-99 */
+void orc_dipole_correction(int n, const double *x, const double *q, const double *cycle, double *trg);
+/* Ewald::wavePart + realPart + selfTerm, include/ewald.h; trg += (4 per body).
+ * Real-space pairs by minimum image (the reference's cell search targets it). */
+void orc_ewald(int n, const double *x, const double *q, int ksize, double alpha, double sigma, double cutoff,
+               const double *cycle, double *trg);
+
 /* Dataset generators (include/dataset.h) and the configured inputs. */
 /* cube(): x in [-pi,pi]^3, dataset.h:52-66 */
 void orc_data_cube(int n, int seed, double *x);

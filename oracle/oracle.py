@@ -76,6 +76,11 @@ def lib():
         L.orc_fmm.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.orc_fmm_periodic.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        _dp, _dp]
+        L.orc_wrap.argtypes = [_dp, _dp]
+        L.orc_wrap.restype = ctypes.c_int
+        L.orc_dipole_correction.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_ewald.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, _dp, _dp]
         L.orc_eval_lists.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_int64, _ip, _ip, _ip, _dp,
                                      _dp, _dp]
         L.orc_upward.argtypes = [_vp, _vp, _dp, _dp, _dp]
@@ -318,6 +323,37 @@ def direct(kernel, xi, xj, qj):
     xi, xj, qj = _c(xi), _c(xj), _c(qj)
     trg = np.zeros((len(xi), 4))
     lib().orc_direct(kernel.h, len(xi), _ptr(xi), _ptr(trg), len(xj), _ptr(xj), _ptr(qj))
+    return trg
+
+
+def wrap(x, cycle):
+    """vec.h wrap of each row of x into [-cycle/2, cycle/2] (a copy) and the moved-axes masks."""
+    x = np.array(x, dtype=np.float64, order="C").reshape(-1, 3)
+    cyc = _c(np.broadcast_to(np.asarray(cycle, dtype=np.float64), (3,)))
+    iw = np.zeros(len(x), np.int32)
+    for i in range(len(x)):
+        row = x[i]
+        buf = (ctypes.c_double * 3)(*row)
+        iw[i] = lib().orc_wrap(buf, _ptr(cyc))
+        x[i] = buf[:]
+    return x, iw
+
+
+def dipole_correction(x, q, cycle, trg):
+    """UpDownPass::getDipole + dipoleCorrection (include/up_down_pass.h:77-99), in place."""
+    x, q = _c(x), _c(q)
+    cyc = _c(np.broadcast_to(np.asarray(cycle, dtype=np.float64), (3,)))
+    assert trg.flags.c_contiguous and trg.dtype == np.float64
+    lib().orc_dipole_correction(len(q), _ptr(x), _ptr(q), _ptr(cyc), _ptr(trg))
+    return trg
+
+
+def ewald(x, q, ksize, alpha, sigma, cutoff, cycle):
+    """Ewald wave + real + self parts (include/ewald.h), FP64; returns (n, 4)."""
+    x, q = _c(x), _c(q)
+    cyc = _c(np.broadcast_to(np.asarray(cycle, dtype=np.float64), (3,)))
+    trg = np.zeros((len(q), 4))
+    lib().orc_ewald(len(q), _ptr(x), _ptr(q), ksize, alpha, sigma, cutoff, _ptr(cyc), _ptr(trg))
     return trg
 
 

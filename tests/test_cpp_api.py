@@ -78,6 +78,26 @@ def test_cpp_args_rejects_bad_distribution(tree_dump):
     assert r.returncode != 0
 
 
+def test_cpp_dipole_correction_matches_oracle(tmp_path):
+    """UpDownPass::getDipole + dipoleCorrection (include/up_down_pass.h:77-99) on host Bodies."""
+    F = _F()
+    exe = str(tmp_path / "dipole_check")
+    libdir = os.path.dirname(F.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dipole_check.cpp"), "-o", exe,
+                    "-L", libdir, "-lexafmm_amd", "-Wl,-rpath," + libdir], check=True)
+    rng = np.random.default_rng(0)
+    n = 300
+    x = rng.random((n, 3)) * 2 * np.pi - np.pi
+    q = rng.random(n) - 0.3
+    trg = rng.random((n, 4))
+    inp = f"{n}\n" + "\n".join(" ".join(repr(float(v)) for v in (*x[i], q[i], *trg[i])) for i in range(n))
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout
+    got = np.array([l.split() for l in out.strip().split("\n")], float)
+    ref = O.dipole_correction(x, q, 2 * np.pi, trg.copy())
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-13)
+
+
 @pytest.mark.gpu
 def test_example_driver_prints_reference_errors():
     """examples/fmm.cxx defaults on 32768 cube bodies print

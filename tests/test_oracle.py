@@ -303,3 +303,91 @@ def test_periodic_supercell_level_adds_its_boxes():
     near = t.fmm_periodic(k, xs, qs, theta, 5000, 1, cyc)
     far = _image_sum(k, xs, qs, shifts[27:])
     assert _rel((trg - near)[:, 0], far[:, 0]) < 3e-5 and _rel((trg - near)[:, 1:], far[:, 1:]) < 3e-5
+
+
+# ---------------------------------------------------------------- Ewald and dipole correction
+MADELUNG_NACL = 1.747564594633182  # rock salt, per ion, nearest-neighbour distance 1
+
+
+def _rock_salt(m):
+    """m^3 ions of alternating charge on the unit lattice, box m (m even), D = 0"""
+    g = np.arange(m)
+    i, j, k = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([i, j, k], -1).reshape(-1, 3).astype(float) - (m - 1) / 2
+    q = np.where((i + j + k).reshape(-1) % 2 == 0, 1.0, -1.0)
+    return x, q
+
+
+def test_ewald_rock_salt_madelung_constant():
+    """Ewald potential of the rock-salt lattice: q_i phi_i = -M for every ion."""
+    x, q = _rock_salt(4)
+    L = 4.0
+    assert np.allclose((x * q[:, None]).sum(0), 0)
+    trg = O.ewald(x, q, ksize=24, alpha=3.0, sigma=0.25 / np.pi, cutoff=L / 2, cycle=L)
+    assert np.max(np.abs(q * trg[:, 0] + MADELUNG_NACL)) < 1e-10
+    assert np.max(np.abs(trg[:, 1:])) < 1e-10  # every ion is a centre of symmetry
+
+
+def _neutral(n, L, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 3)) * L - L / 2
+    q = rng.random(n) - 0.5
+    return x, q - q.mean()
+
+
+def test_ewald_split_parameter_does_not_matter():
+    """alpha only moves work between the real and wave sums (both converged)."""
+    L = 2 * np.pi
+    x, q = _neutral(150, L, 1)
+    a = O.ewald(x, q, 17, 10 / L, 0.25 / np.pi, L / 2, L)
+    b = O.ewald(x, q, 21, 12 / L, 0.25 / np.pi, L / 2, L)
+    assert _rel(a[:, 0], b[:, 0]) < 1e-9 and _rel(a[:, 1:], b[:, 1:]) < 1e-9
+
+
+def test_ewald_gradient_is_energy_gradient():
+    """dE/dx_i = q_i grad phi_i for E = 1/2 sum q phi (central differences)."""
+    L = 2 * np.pi
+    x, q = _neutral(40, L, 2)
+    args = (14, 8 / L, 0.25 / np.pi, L / 2, L)
+    g = O.ewald(x, q, *args)[:, 1:]
+    h = 1e-5
+    for i in (0, 17, 33):
+        for d in range(3):
+            e = []
+            for s in (h, -h):
+                xp = x.copy()
+                xp[i, d] += s
+                e.append(0.5 * q @ O.ewald(xp, q, *args)[:, 0])
+            fd = (e[0] - e[1]) / (2 * h)
+            assert abs(fd - q[i] * g[i, d]) < 1e-6 * max(1.0, abs(q[i] * g[i, d]))
+
+
+def test_dipole_correction_turns_lattice_sum_into_ewald():
+    """A cube of (2K+1)^3 images (vacuum boundary) minus the dipole term of
+    include/up_down_pass.h:89-99 approaches the tin-foil Ewald sum as O(1/K^2);
+    without the correction it does not."""
+    L = 2 * np.pi
+    x, q = _neutral(50, L, 3)
+    ew = O.ewald(x, q, 17, 10 / L, 0.25 / np.pi, L / 2, L)
+    k = O.Kernel(4)
+    K = 8
+    s = np.zeros((len(q), 4))
+    rng = range(-K, K + 1)
+    for a in rng:
+        for b in rng:
+            for c in rng:
+                s += O.direct(k, x, x + np.array([a, b, c]) * L, q)
+    raw = s.copy()
+    O.dipole_correction(x, q, L, s)
+    energy = lambda t: q @ t[:, 0]
+    assert abs(energy(s) - energy(ew)) < 2e-3 * abs(energy(ew))
+    assert _rel(s[:, 1:], ew[:, 1:]) < 3e-4
+    assert _rel(raw[:, 1:], ew[:, 1:]) > 1e-2
+
+
+def test_wrap_into_box():
+    L = np.array([2.0, 3.0, 4.0])
+    x = np.array([[0.5, -1.2, 2.5], [-1.5, 1.6, -2.1], [1.0, 1.5, 2.0]])
+    w, iw = O.wrap(x, L)
+    assert np.allclose(w, [[0.5, -1.2, -1.5], [0.5, -1.4, 1.9], [1.0, 1.5, 2.0]])
+    assert list(iw) == [4, 7, 0]
