@@ -66,6 +66,81 @@ SERVE = {
                        precision="mixed", log2n=(12, 18)),
 }
 
+# Periodic MD stream: a cubic box of side 32 holding 32768 three-site water
+# molecules (O -0.834, H +0.417; O-H 0.308 box units = 0.9572 A with the box
+# taken as 99.4 A, the side of 32768 waters at liquid density; H-O-H 104.52
+# deg; oxygen positions uniform, orientations random), images = 3, P = 10,
+# theta = 0.35; successive steps displace every site by N(0, (0.002 L)^2).
+MD = {
+    "md-water98k-p10": dict(molecules=32768, box=32.0, bond=0.308, angle_deg=104.52, images=3, P=10,
+                            theta=0.35, ncrit=64, nspawn=5000, sigma=0.002, requests=1000),
+}
+
+
+def water_box(w, seed=0):
+    rng = np.random.default_rng(seed)
+    m, L = w["molecules"], w["box"]
+    o = (rng.random((m, 3)) - 0.5) * L
+    u1 = rng.normal(size=(m, 3))
+    u1 /= np.linalg.norm(u1, axis=1, keepdims=True)
+    v = rng.normal(size=(m, 3))
+    v -= (v * u1).sum(1, keepdims=True) * u1
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    a = math.radians(w["angle_deg"])
+    u2 = math.cos(a) * u1 + math.sin(a) * v
+    x = np.stack([o, o + w["bond"] * u1, o + w["bond"] * u2], 1).reshape(-1, 3)
+    q = np.tile([-0.834, 0.417, 0.417], m)
+    return np.ascontiguousarray(x), q
+
+
+def md_bench(F, name, nreq, log):
+    """Successive MD steps through fmm_md_evaluate (wrap, tree, lists, periodic
+    FMM, dipole correction; results in the caller's order): per-step latency
+    includes the host plan, as every step moves every site."""
+    w = MD[name]
+    nreq = nreq or w["requests"]
+    x0, q = water_box(w)
+    n = len(q)
+    rng = np.random.default_rng(1)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        f = F.FMM(P=w["P"], theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], images=w["images"],
+                  cycle=w["box"], precision=F.FMM_MIXED if prec == "mixed" else F.FMM_FP64)
+        x = x0.copy()
+        f.md_evaluate(x, q)  # warm-up
+        lat, dev, host = [], [], []
+        first = None
+        t0 = time.perf_counter()
+        for k in range(nreq):
+            a = time.perf_counter()
+            p, fo = f.md_evaluate(x, q)
+            lat.append(time.perf_counter() - a)
+            s = f.stats()
+            dev.append(s["dev_total_ms"])
+            host.append(s["host_bounds_ms"] + s["host_tree_ms"] + s["host_lists_ms"] + s["host_upload_ms"])
+            if first is None:
+                first = (p.copy(), fo.copy())
+            x += rng.normal(scale=w["sigma"] * w["box"], size=x.shape)
+        wall = time.perf_counter() - t0
+        f.close()
+        lat = np.array(lat)
+        out[prec] = {"particles_per_s": n * nreq / wall, "ms_per_step": 1e3 * wall / nreq,
+                     "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
+                                    "p99": 1e3 * float(np.percentile(lat, 99))},
+                     "device_ms_mean": float(np.mean(dev)), "host_plan_ms_mean": float(np.mean(host)),
+                     "energy_first_step": float(q @ first[0]), "_first": first}
+        log("[md] %s %s: %d steps, %.2f ms/step (device %.2f ms, host plan %.2f ms), p99 %.2f ms"
+            % (name, prec, nreq, out[prec]["ms_per_step"], out[prec]["device_ms_mean"],
+               out[prec]["host_plan_ms_mean"], out[prec]["latency_ms"]["p99"]))
+    p64, f64 = out["fp64"].pop("_first")
+    p32, f32 = out["mixed"].pop("_first")
+    out["mixed_vs_fp64_first_step"] = {
+        "energy_rel": abs(q @ p32 - q @ p64) / abs(q @ p64),
+        "acc_rel_l2": float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))}
+    out["config"] = {"workload": name, "sites": n, **{k: v for k, v in w.items() if k != "requests"},
+                     "requests": nreq}
+    return out
+
 
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB through a copy kernel: the known
@@ -297,8 +372,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plummer1m-p10-fp64", choices=sorted(WORKLOADS) + sorted(SERVE))
-    ap.add_argument("--requests", type=int, default=0, help="serve-*: number of requests (default 2048)")
+    ap.add_argument("--workload", default="plummer1m-p10-fp64",
+                    choices=sorted(WORKLOADS) + sorted(SERVE) + sorted(MD))
+    ap.add_argument("--requests", type=int, default=0,
+                    help="serve-*: number of requests (default 2048); md-*: number of steps (default 1000)")
     ap.add_argument("--m2l-mode", type=int, default=0, choices=(0, 1, 2),
                     help="M2L execution: 0 auto (target tiles), 1 per target, 2 grouped chunks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -318,6 +395,14 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     import exafmm_amd as F
+    if args.workload in MD:
+        r = md_bench(F, args.workload, args.requests, log)
+        out = {"metric": METRIC, "value": r["fp64"]["particles_per_s"], "unit": "particles/s",
+               "n_gpus": 1, "steps": r["config"]["requests"], "warmup": 1,
+               "ms_per_step": r["fp64"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": r.pop("config"), **r}
+        print(json.dumps(out))
+        return
     if args.workload in SERVE:
         r = serve_bench(F, args.workload, args.requests, 64, log)
         out = {"metric": METRIC, "value": r.pop("value"), "unit": r.pop("unit"), "n_gpus": 1,

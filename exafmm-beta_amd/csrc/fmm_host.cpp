@@ -400,50 +400,69 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3],
   const int nth = t.n >= 50000 ? host_threads() : 1;
   std::deque<Walker> ws;  // segments in emission order (stable addresses)
   const int nimg = images > 0 ? 27 : 1;
-  for (int img = 0; img < nimg; img++) {
-  double xp[3] = {0, 0, 0};
-  if (images > 0)
-    image_shift(img, cycle, xp);
-  else if (xper)
-    std::memcpy(xp, xper, sizeof xp);
-  const int tag = images > 0 ? img << kImageShift : 0;
+  auto shift_of = [&](int img, double xp[3]) {
+    xp[0] = xp[1] = xp[2] = 0.0;
+    if (images > 0)
+      image_shift(img, cycle, xp);
+    else if (xper)
+      std::memcpy(xp, xper, sizeof(double) * 3);
+  };
   if (nth == 1) {
-    ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
-    Walker &w = ws.back();
-    w.tag = tag;
-    w.m2l_i.reserve(64 * (size_t)t.ncell / nimg);
-    w.m2l_j.reserve(64 * (size_t)t.ncell / nimg);
-    w.p2p_i.reserve(32 * (size_t)t.leaves.size() / nimg);
-    w.p2p_j.reserve(32 * (size_t)t.leaves.size() / nimg);
-    w.dual(0, 0);
-  } else {
-    // The recursion down to cell pairs of at most `cut` bodies runs here and
-    // leaves the pairs below it as tasks in visiting order; threads traverse
-    // blocks of consecutive tasks, and the blocks' pairs are taken in block
-    // order, which is the serial visiting order.
-    std::vector<std::pair<int, int>> tasks;
-    Walker top{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}};
-    top.cut = std::max(2 * 64, t.n / (64 * nth));
-    top.tasks = &tasks;
-    top.dual(0, 0);
-    const size_t nblk = std::min<size_t>(tasks.size(), (size_t)nth * 16);
-    const size_t first = ws.size();
-    for (size_t b = 0; b < nblk; b++) {
+    for (int img = 0; img < nimg; img++) {
+      double xp[3];
+      shift_of(img, xp);
       ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
-      ws.back().tag = tag;
+      Walker &w = ws.back();
+      w.tag = images > 0 ? img << kImageShift : 0;
+      w.m2l_i.reserve(64 * (size_t)t.ncell / nimg);
+      w.m2l_j.reserve(64 * (size_t)t.ncell / nimg);
+      w.p2p_i.reserve(32 * (size_t)t.leaves.size() / nimg);
+      w.p2p_j.reserve(32 * (size_t)t.leaves.size() / nimg);
+      w.dual(0, 0);
+    }
+  } else {
+    // For every image, the recursion down to cell pairs of at most `cut`
+    // bodies runs here and leaves the pairs below it as tasks in visiting
+    // order; one pool of threads then traverses blocks of consecutive tasks
+    // of all images, and the blocks' pairs are taken in (image, block)
+    // order, which is the serial visiting order.
+    std::vector<std::vector<std::pair<int, int>>> tasks(nimg);
+    size_t ntask = 0;
+    for (int img = 0; img < nimg; img++) {
+      double xp[3];
+      shift_of(img, xp);
+      Walker top{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}};
+      top.cut = std::max(2 * 64, t.n / (8 * nth));
+      top.tasks = &tasks[img];
+      top.dual(0, 0);
+      ntask += tasks[img].size();
+    }
+    struct Blk {
+      int img;
+      size_t lo, hi;
+    };
+    std::vector<Blk> blk;
+    const size_t per = std::max<size_t>(1, ntask / ((size_t)nth * 16));
+    for (int img = 0; img < nimg; img++) {
+      double xp[3];
+      shift_of(img, xp);
+      for (size_t lo = 0; lo < tasks[img].size(); lo += per) {
+        blk.push_back({img, lo, std::min(tasks[img].size(), lo + per)});
+        ws.push_back(Walker{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}});
+        ws.back().tag = images > 0 ? img << kImageShift : 0;
+      }
     }
     std::atomic<size_t> next{0};
     auto work = [&] {
-      for (size_t b; (b = next.fetch_add(1)) < nblk;) {
-        const size_t lo = tasks.size() * b / nblk, hi = tasks.size() * (b + 1) / nblk;
-        for (size_t k = lo; k < hi; k++) ws[first + b].dual(tasks[k].first, tasks[k].second);
+      for (size_t b; (b = next.fetch_add(1)) < blk.size();) {
+        const auto &tk = tasks[blk[b].img];
+        for (size_t k = blk[b].lo; k < blk[b].hi; k++) ws[b].dual(tk[k].first, tk[k].second);
       }
     };
     std::vector<std::thread> th;
     for (int i = 1; i < nth; i++) th.emplace_back(work);
     work();
     for (auto &x : th) x.join();
-  }
   }
   std::vector<Seg> sm, sp;
   for (const Walker &w : ws) {
@@ -564,70 +583,104 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   }
   const double R0 = t.R[0];
   const int64_t np = L.nm2l;
-  std::vector<int> gid(np);
-  std::vector<int64_t> kvec;   // 3 per group
-  std::vector<int> gexp;       // e per group
-  std::vector<int> gimg;       // image per group
-  std::vector<double> sc;      // by_level: (R_s, R_t) per group
-  KeyIndex2 index;
   const int64_t lim = 1 << 18;
-  double pow2[64];             // 2^e / R0
+  double pow2[64];  // 2^e / R0
   for (int e = 0; e < 64; e++) pow2[e] = std::ldexp(1.0, e) / R0;
   // round to nearest (ties to even) for |v| < 2^51 without a libm call
   auto rnd = [](double v) { return (v + 6755399441055744.0) - 6755399441055744.0; };
-  for (int c = 0; c < nc; c++) {
-    const double xc = t.X[3 * (size_t)c], yc = t.X[3 * (size_t)c + 1], zc = t.X[3 * (size_t)c + 2];
-    const int lt = t.level[c];
-    for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
-      const int scell = cell_of(L.m2l_src[p]), img = image_of(L.m2l_src[p]);
-      const int ls = t.level[scell];
-      int e = std::max(lt, ls);
-      int64_t k[3];
-      bool ok = e < 64;
-      if (ok) {
-        const double scale = pow2[e];
-        const double v[3] = {(xc - t.X[3 * (size_t)scell]) * scale, (yc - t.X[3 * (size_t)scell + 1]) * scale,
-                             (zc - t.X[3 * (size_t)scell + 2]) * scale};
-        for (int d = 0; d < 3; d++) {
-          const double r = rnd(v[d]);
-          if (!(std::fabs(v[d] - r) < 1e-6) || std::fabs(r) >= (double)(lim << 8)) ok = false;
-          k[d] = (int64_t)r;
+  // Runs of target cells with about equal pair counts, one per thread.  Each
+  // run numbers the keys it meets in first-seen order; merging the runs in
+  // order gives every group the index a single pass in CSR order would give.
+  const int T = np >= 200000 ? host_threads() : 1;
+  std::vector<int> cut(T + 1, nc);
+  cut[0] = 0;
+  for (int r = 1; r < T; r++)
+    cut[r] = (int)(std::lower_bound(L.m2l_ptr.begin(), L.m2l_ptr.end(), (int)(np * r / T)) - L.m2l_ptr.begin());
+  for (int r = 1; r <= T; r++) cut[r] = std::max(cut[r], cut[r - 1]);
+  struct Fresh {
+    uint64_t key, key2;
+    int64_t k[3];
+    int e, img;
+    double rs, rt;
+  };
+  std::vector<int> lid(np);  // run-local group of each pair, -1: not grouped
+  std::vector<std::vector<Fresh>> fresh(T);
+  auto par = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (int r = 1; r < T; r++) th.emplace_back(fn, r);
+    fn(0);
+    for (auto &x : th) x.join();
+  };
+  par([&](int r) {
+    KeyIndex2 index;
+    std::vector<Fresh> &fr = fresh[r];
+    for (int c = cut[r]; c < cut[r + 1]; c++) {
+      const double xc = t.X[3 * (size_t)c], yc = t.X[3 * (size_t)c + 1], zc = t.X[3 * (size_t)c + 2];
+      const int lt = t.level[c];
+      for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+        const int scell = cell_of(L.m2l_src[p]), img = image_of(L.m2l_src[p]);
+        const int ls = t.level[scell];
+        int e = std::max(lt, ls);
+        int64_t k[3];
+        bool ok = e < 64;
+        if (ok) {
+          const double scale = pow2[e];
+          const double v[3] = {(xc - t.X[3 * (size_t)scell]) * scale, (yc - t.X[3 * (size_t)scell + 1]) * scale,
+                               (zc - t.X[3 * (size_t)scell + 2]) * scale};
+          for (int d = 0; d < 3; d++) {
+            const double rv = rnd(v[d]);
+            if (!(std::fabs(v[d] - rv) < 1e-6) || std::fabs(rv) >= (double)(lim << 8)) ok = false;
+            k[d] = (int64_t)rv;
+          }
+        }
+        if (ok) {
+          while (e > 0 && ((k[0] | k[1] | k[2]) & 1) == 0) {
+            k[0] /= 2;
+            k[1] /= 2;
+            k[2] /= 2;
+            e--;
+          }
+          for (int d = 0; d < 3; d++)
+            if (k[d] <= -lim || k[d] >= lim) ok = false;
+        }
+        if (!ok) {
+          lid[p] = -1;
+          continue;
+        }
+        const uint64_t key = ((uint64_t)e << 57) | ((uint64_t)(k[0] + lim) << 38) |
+                             ((uint64_t)(k[1] + lim) << 19) | (uint64_t)(k[2] + lim);
+        // the image, and (by_level) the source and target levels, are part of the group
+        const uint64_t key2 = ((uint64_t)img << 20) | (by_level ? (((uint64_t)lt << 8) | (uint64_t)ls) + 1 : 0);
+        bool isnew;
+        lid[p] = index.get(key, key2, isnew);
+        if (isnew) fr.push_back({key, key2, {k[0], k[1], k[2]}, e, img, t.R[scell], t.R[c]});
+      }
+    }
+  });
+  // merge in run order: global group numbers, offsets and (by_level) radii
+  std::vector<std::vector<int>> gmap(T);
+  std::vector<double> sc;  // by_level: (R_s, R_t) of each group's first pair
+  std::vector<int64_t> kvec;
+  std::vector<int> gexp, gimg;
+  {
+    KeyIndex2 index;
+    for (int r = 0; r < T; r++) {
+      gmap[r].resize(fresh[r].size());
+      for (size_t u = 0; u < fresh[r].size(); u++) {
+        const Fresh &f = fresh[r][u];
+        bool isnew;
+        const int g = index.get(f.key, f.key2, isnew);
+        gmap[r][u] = g;
+        if (isnew) {
+          kvec.insert(kvec.end(), {f.k[0], f.k[1], f.k[2]});
+          gexp.push_back(f.e);
+          gimg.push_back(f.img);
+          sc.insert(sc.end(), {f.rs, f.rt});
         }
       }
-      if (ok) {
-        while (e > 0 && ((k[0] | k[1] | k[2]) & 1) == 0) {
-          k[0] /= 2;
-          k[1] /= 2;
-          k[2] /= 2;
-          e--;
-        }
-        for (int d = 0; d < 3; d++)
-          if (k[d] <= -lim || k[d] >= lim) ok = false;
-      }
-      if (!ok) {
-        gid[p] = -1;
-        continue;
-      }
-      const uint64_t key = ((uint64_t)e << 57) | ((uint64_t)(k[0] + lim) << 38) |
-                           ((uint64_t)(k[1] + lim) << 19) | (uint64_t)(k[2] + lim);
-      // the image, and (by_level) the source and target levels, are part of the group
-      const uint64_t key2 = ((uint64_t)img << 20) | (by_level ? (((uint64_t)lt << 8) | (uint64_t)ls) + 1 : 0);
-      bool fresh;
-      const int g = index.get(key, key2, fresh);
-      if (fresh) {
-        kvec.insert(kvec.end(), {k[0], k[1], k[2]});
-        gexp.push_back(e);
-        gimg.push_back(img);
-        if (by_level) sc.insert(sc.end(), {t.R[scell], t.R[c]});
-      }
-      if (by_level && (t.R[scell] != sc[2 * (size_t)g] || t.R[c] != sc[2 * (size_t)g + 1])) {
-        gid[p] = -1;  // radius not uniform on its level: FP64 per-target path
-        continue;
-      }
-      gid[p] = g;
     }
   }
-  const int ng = index.count;
+  const int ng = (int)gexp.size();
   G.dx.resize(3 * (size_t)ng);
   for (int g = 0; g < ng; g++) {
     const double u = R0 / std::ldexp(1.0, gexp[g]);
@@ -635,33 +688,61 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     if (cycle) image_shift(gimg[g], cycle, sh);
     for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u - sh[d];
   }
-  if (by_level) G.scale = std::move(sc);
   G.ngroup = ng;
-  // counting sort by group, CSR order kept inside a group
+  // global group per pair; by_level: a pair whose radii differ from its
+  // group's first pair runs per target (radius not uniform on its level);
+  // counts per (run, group) and per target of the ungrouped rest
+  std::vector<std::vector<int>> cnt(T);
+  par([&](int r) {
+    std::vector<int> &cn = cnt[r];
+    cn.assign(ng, 0);
+    const std::vector<int> &m = gmap[r];
+    for (int c = cut[r]; c < cut[r + 1]; c++)
+      for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+        int g = lid[p] >= 0 ? m[lid[p]] : -1;
+        if (g >= 0 && by_level &&
+            (t.R[cell_of(L.m2l_src[p])] != sc[2 * (size_t)g] || t.R[c] != sc[2 * (size_t)g + 1]))
+          g = -1;
+        lid[p] = g;
+        if (g >= 0)
+          cn[g]++;
+        else
+          G.rest_ptr[c + 1]++;
+      }
+  });
+  if (by_level) G.scale = std::move(sc);
+  // counting sort by group, CSR order kept inside a group (runs in order)
   G.grp_ptr.assign(ng + 1, 0);
-  for (int64_t p = 0; p < np; p++)
-    if (gid[p] >= 0) G.grp_ptr[gid[p] + 1]++;
-  for (int g = 0; g < ng; g++) G.grp_ptr[g + 1] += G.grp_ptr[g];
+  for (int g = 0; g < ng; g++) {
+    int64_t a = G.grp_ptr[g];
+    for (int r = 0; r < T; r++) {
+      const int k = cnt[r][g];
+      cnt[r][g] = (int)a;  // run r's first slot in group g
+      a += k;
+    }
+    G.grp_ptr[g + 1] = (int)a;
+  }
+  for (int c = 0; c < nc; c++) G.rest_ptr[c + 1] += G.rest_ptr[c];
   const int64_t ngp = G.grp_ptr[ng];
   G.tgt.resize(ngp);
   G.src.resize(ngp);
-  std::vector<int> fill(G.grp_ptr.begin(), G.grp_ptr.end() - 1);
-  std::vector<int> rest_t, rest_s;
-  for (int c = 0; c < nc; c++) {
-    for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
-      const int g = gid[p];
-      if (g >= 0) {
-        const int q = fill[g]++;
-        G.tgt[q] = c;
-        G.src[q] = L.m2l_src[p];
-      } else {
-        G.rest_ptr[c + 1]++;
-        rest_s.push_back(L.m2l_src[p]);
+  G.rest_src.resize(G.rest_ptr[nc]);
+  par([&](int r) {
+    std::vector<int> &fill = cnt[r];
+    for (int c = cut[r]; c < cut[r + 1]; c++) {
+      int rq = G.rest_ptr[c];
+      for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
+        const int g = lid[p];
+        if (g >= 0) {
+          const int q = fill[g]++;
+          G.tgt[q] = c;
+          G.src[q] = L.m2l_src[p];
+        } else {
+          G.rest_src[rq++] = L.m2l_src[p];
+        }
       }
     }
-  }
-  for (int c = 0; c < nc; c++) G.rest_ptr[c + 1] += G.rest_ptr[c];
-  G.rest_src = std::move(rest_s);
+  });
   G.nrest = (int64_t)G.rest_src.size();
   for (int c = 0; c < nc; c++)
     if (G.rest_ptr[c + 1] > G.rest_ptr[c]) G.rest_targets.push_back(c);
