@@ -376,8 +376,8 @@ def main():
                     choices=sorted(WORKLOADS) + sorted(SERVE) + sorted(MD))
     ap.add_argument("--requests", type=int, default=0,
                     help="serve-*: number of requests (default 2048); md-*: number of steps (default 1000)")
-    ap.add_argument("--m2l-mode", type=int, default=0, choices=(0, 1, 2),
-                    help="M2L execution: 0 auto (target tiles), 1 per target, 2 grouped chunks")
+    ap.add_argument("--m2l-mode", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="M2L execution: 0 auto, 1 per target, 2 grouped chunks, 3 target tiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -448,7 +448,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "P": P, "theta": w["theta"],
                    "ncrit": w["ncrit"], "nspawn": w["nspawn"], "inputs": w["desc"],
-                   "precision": w["precision"], "m2l_mode": ("auto", "target", "grouped")[args.m2l_mode]},
+                   "precision": w["precision"], "m2l_mode": ("auto", "target", "grouped", "tiles")[args.m2l_mode]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(r["lat"], 50)),
                        "p99": 1e3 * float(np.percentile(r["lat"], 99))},
         "device_ms": {"total": float(np.mean(r["dev"])), "upward": st["dev_upward_ms"],

@@ -78,7 +78,7 @@ struct fmm_ctx {
   int nchunk = 0, ntile = 0;
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
-  DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y;
+  DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y, g_ops;
   // M2M / L2L as per-level operators on MFMA (regular trees)
   bool xl = false;
   std::vector<int> xl_tptr;  // per child level: first L2L tile
@@ -211,7 +211,8 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
   } else if (p->images < 0 || (p->images > 0 && !(p->cycle[0] > 0 && p->cycle[1] > 0 && p->cycle[2] > 0))) {
     c->err = "images must be >= 0, and the cycle > 0 in x, y and z when images > 0";
     rc = FMM_ERR_ARG;
-  } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET && p->m2l_mode != FMM_M2L_GROUPED) {
+  } else if (p->m2l_mode != FMM_M2L_AUTO && p->m2l_mode != FMM_M2L_TARGET && p->m2l_mode != FMM_M2L_GROUPED &&
+             p->m2l_mode != FMM_M2L_TILES) {
     c->err = "invalid m2l_mode";
     rc = FMM_ERR_ARG;
   } else if (p->precision != FMM_FP64 && p->precision != FMM_MIXED) {
@@ -357,7 +358,9 @@ int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
       pl->tiles = fmm::M2LTiles();
     }
     const int TT = fmmdev::m2l_tile_targets(p->P);
-    if (p->m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(p->P) && pl->tiles.TT != TT)
+    const bool tiles =
+        p->m2l_mode == FMM_M2L_TILES || (p->m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(p->P, mixed));
+    if (tiles && pl->tiles.TT != TT)
       pl->tiles = fmm::build_m2l_tiles(pl->tree, pl->groups, TT, 16);
   } catch (const std::bad_alloc &) {
     return FMM_ERR_MEMORY;
@@ -595,7 +598,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   c->mixed = c->p.precision == FMM_MIXED;
   c->use_mfma = c->p.m2l_mode != FMM_M2L_TARGET && L.nm2l > 0 &&
                 (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
-  c->use_tiles = c->use_mfma && c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(c->p.P);
+  c->use_tiles = c->use_mfma && (c->p.m2l_mode == FMM_M2L_TILES ||
+                                 (c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(c->p.P, c->mixed)));
   c->groups = fmm::M2LGroups();
   c->nchunk = 0;
   c->ntile = 0;
@@ -692,11 +696,14 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     HIPCHK(fmmdev::launch_xlate_ops(c->p.P, c->xl_dx.as<double>(), c->xl_kinds.as<int>(), nop,
                                     c->xl_ops.as<double>(), 0, c->st));
   }
-  if (c->use_tiles) {  // harmonics of every group offset, once per plan
+  if (c->use_tiles) {  // harmonics, then the dense operator, of every group offset, once per plan
     const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * c->groups.ngroup;
     if ((rc = ensure(c, c->g_y, sizeof(double2) * ny))) return rc;
     HIPCHK(fmmdev::launch_group_harmonics(c->p.P, c->g_dx.as<double>(), c->groups.ngroup, c->g_y.as<double2>(),
                                           c->st));
+    if ((rc = ensure(c, c->g_ops, fmmdev::group_ops_bytes(c->p.P, c->mixed) * c->groups.ngroup))) return rc;
+    HIPCHK(fmmdev::launch_group_ops(c->p.P, c->mixed, c->g_y.as<double2>(), c->g_sc.as<double2>(),
+                                    c->groups.ngroup, c->g_ops.p, c->st));
   }
   double t4 = now_ms();
   pl.tree = fmm::Tree();
@@ -712,7 +719,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   s.host_upload_ms = t4 - t3;
   s.m2l_groups = c->groups.ngroup;
   s.m2l_ungrouped = c->use_mfma ? c->groups.nrest : L.nm2l;
-  s.m2l_path = c->use_tiles ? FMM_M2L_AUTO : c->use_mfma ? FMM_M2L_GROUPED : FMM_M2L_TARGET;
+  s.m2l_path = c->use_tiles ? FMM_M2L_TILES : c->use_mfma ? FMM_M2L_GROUPED : FMM_M2L_TARGET;
   c->have_bodies = true;
   return FMM_OK;
 }
@@ -857,8 +864,8 @@ int fmm_m2l(fmm_ctx *c) {
     if (c->use_tiles) {
       HIPCHK(fmmdev::launch_m2l_tile(c->p.P, c->mixed, c->ntile, c->t_tgt.as<int>(), c->t_ptr.as<int4>(),
                                      c->t_slice.as<int4>(), c->t_src.as<int>(), c->t_lt.as<unsigned char>(),
-                                     c->g_y.as<double2>(), c->g_sc.as<double2>(), c->cr.as<double>(),
-                                     c->mixed ? c->M32.p : c->M.p, c->L.as<double2>(), c->st));
+                                     c->g_ops.p, c->cr.as<double>(), c->mixed ? c->M32.p : c->M.p,
+                                     c->L.as<double2>(), c->st));
     } else if (c->mixed) {
       HIPCHK(fmmdev::launch_m2l_mfma32(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
                                        c->g_sc.as<double2>(), c->g_tgt.as<int>(), c->g_src.as<int>(),

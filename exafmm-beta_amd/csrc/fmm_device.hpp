@@ -57,10 +57,15 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
 // operands on M32, else FP64 on M.  A tile in one item stores its targets' L,
 // a split tile adds into L (which must start at zero).
 int m2l_tile_targets(int P);
-bool m2l_tiles_preferred(int P);  // AUTO uses tiles (else chunks) at this order
+bool m2l_tiles_preferred(int P, bool mixed);  // AUTO uses tiles (else chunks) at this order
 hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const int4 *items, const int4 *slices,
-                           const int *psrc, const unsigned char *plt, const double2 *gY, const double2 *gscale,
-                           const double *cr, const void *M, double2 *L, hipStream_t s);
+                           const int *psrc, const unsigned char *plt, const void *ops, const double *cr,
+                           const void *M, double2 *L, hipStream_t s);
+// every group's dense operator as the tile kernel's MFMA A operands (from the
+// group harmonics); group_ops_bytes: bytes per group
+size_t group_ops_bytes(int P, bool mix);
+hipError_t launch_group_ops(int P, bool mix, const double2 *gY, const double2 *gscale, int ngroup, void *ops,
+                            hipStream_t s);
 // singular harmonics of every group offset (group_harmonics_size(P) complex each)
 int group_harmonics_size(int P);
 hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 *Y, hipStream_t s);

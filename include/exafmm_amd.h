@@ -44,12 +44,14 @@ typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
  * target cells and sums all their groups in LDS before one store (a tile too
  * long for one block is split and its parts are added atomically).  Chunks:
  * one block per chunk of <= 512 pairs of one group, results added with FP64
- * atomics.  AUTO (P <= 12) uses tiles for P <= 8 and chunks above (measured:
- * atomics dominate at low order, operator builds at high order); GROUPED
- * always uses chunks; both repeat to rounding.  TARGET: one wavefront per
+ * atomics.  AUTO (P <= 12) uses tiles for FP64 P <= 10 and every mixed order,
+ * chunks for FP64 P = 11, 12 (measured: the atomics cost more than the
+ * tiles' operator reads except for the FP64 tables of the highest orders); GROUPED
+ * always uses chunks, TILES always tiles; all repeat to rounding.  The m2l_path
+ * of fmm_stats reports TILES or GROUPED for the layout taken.  TARGET: one wavefront per
  * target cell on vector ALUs (fixed summation order, bitwise repeatable; any
  * P).  Pairs whose offset does not fit a group run per target in every mode. */
-typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1, FMM_M2L_GROUPED = 2 } fmm_m2l_mode;
+typedef enum { FMM_M2L_AUTO = 0, FMM_M2L_TARGET = 1, FMM_M2L_GROUPED = 2, FMM_M2L_TILES = 3 } fmm_m2l_mode;
 
 /* The knobs of exafmm-beta's Kernel / Traversal / BuildTree constructors and
  * Args defaults (include/args.h:188-205): P=10, theta=0.4, ncrit=64,

@@ -36,14 +36,15 @@ def _solve(F, x, q, P, precision, **kw):
 
 @pytest.mark.parametrize("P,dist,n,mode", [(4, "cube1", 20000, 0), (8, "cube1", 40000, 0),
                                            (10, "plummer1", 40000, 0), (12, "cube1", 40000, 0),
-                                           (8, "cube1", 40000, 2), (12, "plummer1", 30000, 2)])
+                                           (8, "cube1", 40000, 2), (12, "plummer1", 30000, 2),
+                                           (10, "cube1", 40000, 3), (12, "plummer1", 30000, 3)])
 def test_mixed_matches_fp64(P, dist, n, mode):
     F = _F()
     x, q = F.make_bodies(n, dist, 1)
     t64, s64 = _solve(F, x, q, P, F.FMM_FP64)
     t32, s32 = _solve(F, x, q, P, F.FMM_MIXED, m2l_mode=mode)
-    # AUTO runs target tiles up to P = 8 and chunks above
-    assert s32["m2l_path"] == (mode if mode != F.M2L_AUTO or P <= 8 else F.M2L_GROUPED)
+    # AUTO runs mixed M2L in target tiles at every order
+    assert s32["m2l_path"] == (mode if mode != F.M2L_AUTO else F.M2L_TILES)
     assert s32["m2l_groups"] > 0 and s32["m2l_ungrouped"] == 0
     dp, da = _rel(t32[:, 0], t64[:, 0]), _rel(t32[:, 1:], t64[:, 1:])
     idx = np.arange(1000) * (n // 1000)

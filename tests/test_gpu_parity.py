@@ -201,7 +201,7 @@ def test_input_order_and_determinism():
     g.evaluate()
     assert _rel(g.targets(input_order=False), c) < 1e-14
     assert _rel(c, a) < 1e-13
-    assert g.stats()["m2l_path"] == F.M2L_AUTO
+    assert g.stats()["m2l_path"] == F.M2L_TILES
     g.close()
     # grouped chunks accumulate with atomics: repeatable to rounding
     h = F.FMM(P=8, m2l_mode=F.M2L_GROUPED)
@@ -220,7 +220,7 @@ def test_grouped_m2l_matches_per_target(P):
     x, q = _inputs(20000, "plummer", seed=4)
     F = _fmm()
     out = []
-    for mode in (F.M2L_TARGET, F.M2L_AUTO, F.M2L_GROUPED):
+    for mode in (F.M2L_TARGET, F.M2L_AUTO, F.M2L_GROUPED, F.M2L_TILES):
         f = F.FMM(P=P, m2l_mode=mode)
         f.set_bodies(x, q)
         f.upward()
@@ -228,8 +228,9 @@ def test_grouped_m2l_matches_per_target(P):
         out.append(f.expansions()[1])
         st = f.stats()
         f.close()
-    assert st["m2l_groups"] > 0 and st["m2l_ungrouped"] == 0
-    assert _rel(out[1], out[0]) < PHASE_TOL and _rel(out[2], out[0]) < PHASE_TOL
+    assert st["m2l_groups"] > 0 and st["m2l_ungrouped"] == 0 and st["m2l_path"] == F.M2L_TILES
+    for k in (1, 2, 3):
+        assert _rel(out[k], out[0]) < PHASE_TOL
 
 
 def test_set_charges_linearity():
