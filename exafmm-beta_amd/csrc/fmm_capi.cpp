@@ -78,7 +78,7 @@ struct fmm_ctx {
   int nchunk = 0, ntile = 0;
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
-  DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y, g_ops;
+  DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y, g_ops, c_dx, c_sc;
   // M2M / L2L as per-level operators on MFMA (regular trees)
   bool xl = false;
   std::vector<int> xl_tptr;  // per child level: first L2L tile
@@ -618,7 +618,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
       if (tiles.TT == 0) tiles = fmm::build_m2l_tiles(t, G, fmmdev::m2l_tile_targets(c->p.P), 16);
       c->ntile = (int)(tiles.item.size() / 4);  // work items
       UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item) UP(t_slice, tiles.slice) UP(t_src, tiles.src)
-      UP(t_lt, tiles.lt)
+      UP(t_lt, tiles.lt) UP(c_dx, G.cdx)
+      if (c->mixed) UP(c_sc, G.cscale)
     } else {
       c->nchunk = (int)(G.chunk.size() / 4);
       UP(g_tgt, G.tgt) UP(g_src, G.src) UP(g_chunk, G.chunk)
@@ -696,14 +697,14 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     HIPCHK(fmmdev::launch_xlate_ops(c->p.P, c->xl_dx.as<double>(), c->xl_kinds.as<int>(), nop,
                                     c->xl_ops.as<double>(), 0, c->st));
   }
-  if (c->use_tiles) {  // harmonics, then the dense operator, of every group offset, once per plan
-    const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * c->groups.ngroup;
+  if (c->use_tiles) {  // harmonics, then the dense operator, of every representative offset, once per plan
+    const int nr = c->groups.ncanon;
+    const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * nr;
     if ((rc = ensure(c, c->g_y, sizeof(double2) * ny))) return rc;
-    HIPCHK(fmmdev::launch_group_harmonics(c->p.P, c->g_dx.as<double>(), c->groups.ngroup, c->g_y.as<double2>(),
-                                          c->st));
-    if ((rc = ensure(c, c->g_ops, fmmdev::group_ops_bytes(c->p.P, c->mixed) * c->groups.ngroup))) return rc;
-    HIPCHK(fmmdev::launch_group_ops(c->p.P, c->mixed, c->g_y.as<double2>(), c->g_sc.as<double2>(),
-                                    c->groups.ngroup, c->g_ops.p, c->st));
+    HIPCHK(fmmdev::launch_group_harmonics(c->p.P, c->c_dx.as<double>(), nr, c->g_y.as<double2>(), c->st));
+    if ((rc = ensure(c, c->g_ops, fmmdev::group_ops_bytes(c->p.P, c->mixed) * nr))) return rc;
+    HIPCHK(fmmdev::launch_group_ops(c->p.P, c->mixed, c->g_y.as<double2>(), c->c_sc.as<double2>(), nr, c->g_ops.p,
+                                    c->st));
   }
   double t4 = now_ms();
   pl.tree = fmm::Tree();

@@ -662,6 +662,7 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   std::vector<double> sc;  // by_level: (R_s, R_t) of each group's first pair
   std::vector<int64_t> kvec;
   std::vector<int> gexp, gimg;
+  std::vector<uint64_t> gkey2;
   {
     KeyIndex2 index;
     for (int r = 0; r < T; r++) {
@@ -675,6 +676,7 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
           kvec.insert(kvec.end(), {f.k[0], f.k[1], f.k[2]});
           gexp.push_back(f.e);
           gimg.push_back(f.img);
+          gkey2.push_back(f.key2);
           sc.insert(sc.end(), {f.rs, f.rt});
         }
       }
@@ -689,6 +691,41 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u - sh[d];
   }
   G.ngroup = ng;
+  // reflection classes (non-periodic offsets only)
+  G.canon.resize(ng);
+  G.refl.assign(ng, 0);
+  {
+    KeyIndex2 index;
+    for (int g = 0; g < ng; g++) {
+      const int64_t *k = &kvec[3 * (size_t)g];
+      uint8_t r = 0;
+      int64_t a[3];
+      for (int d = 0; d < 3; d++) {
+        a[d] = k[d];
+        if (!cycle && k[d] < 0) {
+          a[d] = -k[d];
+          r |= (uint8_t)(1 << d);
+        }
+      }
+      const uint64_t key = cycle ? (uint64_t)g  // periodic: every group is its own class
+                                 : ((uint64_t)gexp[g] << 57) | ((uint64_t)(a[0] + lim) << 38) |
+                                       ((uint64_t)(a[1] + lim) << 19) | (uint64_t)(a[2] + lim);
+      bool isnew;
+      const int cg = index.get(key, cycle ? 0 : gkey2[g], isnew);
+      G.canon[g] = cg;
+      G.refl[g] = r;
+      if (isnew) {
+        const double u = R0 / std::ldexp(1.0, gexp[g]);
+        if (cycle)
+          G.cdx.insert(G.cdx.end(), {G.dx[3 * (size_t)g], G.dx[3 * (size_t)g + 1], G.dx[3 * (size_t)g + 2]});
+        else
+          G.cdx.insert(G.cdx.end(), {(double)a[0] * u, (double)a[1] * u, (double)a[2] * u});
+        G.cscale.insert(G.cscale.end(), {sc[2 * (size_t)g], sc[2 * (size_t)g + 1]});
+      }
+    }
+    G.ncanon = index.count;
+  }
+  if (!by_level) G.cscale.clear();
   // global group per pair; by_level: a pair whose radii differ from its
   // group's first pair runs per target (radius not uniform on its level);
   // counts per (run, group) and per target of the ungrouped rest
@@ -832,16 +869,22 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
     tile_of[c] = (int)(i / TT);
     local[c] = (int)(i % TT);
   }
-  // pairs bucketed by tile; taking them in group order keeps each tile's
-  // pairs sorted by group
+  // pairs bucketed by tile; taking the groups in (representative,
+  // reflection) order keeps each tile's pairs sorted that way, so a tile
+  // loads each representative's operator once
   std::vector<int64_t> pcount(nt + 1, 0);
   for (int64_t q = 0; q < np; q++) pcount[tile_of[G.tgt[q]] + 1]++;
   for (int i = 0; i < nt; i++) pcount[i + 1] += pcount[i];
   std::vector<int> pgrp(np), psrc(np);
   std::vector<uint8_t> plt(np);
+  std::vector<int> gorder(G.ngroup);
+  for (int g = 0; g < G.ngroup; g++) gorder[g] = g;
+  std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
+    return G.canon[a] != G.canon[b] ? G.canon[a] < G.canon[b] : G.refl[a] < G.refl[b];
+  });
   {
     std::vector<int64_t> fill(pcount.begin(), pcount.end() - 1);
-    for (int g = 0; g < G.ngroup; g++)
+    for (int g : gorder)
       for (int q = G.grp_ptr[g]; q < G.grp_ptr[g + 1]; q++) {
         const int c = G.tgt[q];
         const int64_t k = fill[tile_of[c]]++;
@@ -867,7 +910,7 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
         const int g = pgrp[k];
         int64_t e = k;
         while (e < pcount[tile + 1] && pgrp[e] == g && e - k < SW) e++;
-        T.slice.insert(T.slice.end(), {g, (int)T.src.size(), (int)(e - k), 0});
+        T.slice.insert(T.slice.end(), {G.canon[g], (int)T.src.size(), (int)(e - k), (int)G.refl[g]});
         for (int64_t u = k; u < e; u++) {
           T.src.push_back(psrc[u]);
           T.lt.push_back(plt[u]);
@@ -949,6 +992,11 @@ void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroup
     for (int v : g.src) G->src.push_back(v + co);
     G->dx.insert(G->dx.end(), g.dx.begin(), g.dx.end());
     G->scale.insert(G->scale.end(), g.scale.begin(), g.scale.end());
+    for (int v : g.canon) G->canon.push_back(v + G->ncanon);
+    G->refl.insert(G->refl.end(), g.refl.begin(), g.refl.end());
+    G->cdx.insert(G->cdx.end(), g.cdx.begin(), g.cdx.end());
+    G->cscale.insert(G->cscale.end(), g.cscale.begin(), g.cscale.end());
+    G->ncanon += g.ncanon;
     const int ro = (int)G->rest_src.size();
     for (int c = 0; c < p.t->ncell; c++) G->rest_ptr[co + c + 1] = g.rest_ptr[c + 1] + ro;
     for (int v : g.rest_src) G->rest_src.push_back(v + co);
@@ -974,12 +1022,12 @@ void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroup
     for (int v : a.tgt) X.tgt.push_back(v >= 0 ? v + co : -1);
     for (int b = 0; b < a.ntile; b++) X.ptr.push_back(a.ptr[b + 1] + so);
     for (size_t q = 0; q < a.slice.size(); q += 4)
-      X.slice.insert(X.slice.end(), {a.slice[q] + go, a.slice[q + 1] + po2, a.slice[q + 2], 0});
+      X.slice.insert(X.slice.end(), {a.slice[q] + go, a.slice[q + 1] + po2, a.slice[q + 2], a.slice[q + 3]});
     for (int v : a.src) X.src.push_back(v + co);
     X.lt.insert(X.lt.end(), a.lt.begin(), a.lt.end());
     X.ntile += a.ntile;
     co += p.t->ncell;
-    go += p.g->ngroup;
+    go += p.g->ncanon;
   }
   int seg = 1;
   for (const PlanPart &p : parts)

@@ -94,6 +94,18 @@ struct M2LGroups {
   std::vector<int> rest_ptr, rest_src, rest_targets;
   int64_t nrest = 0;
   std::vector<double> scale;       // 2 per group (R_s, R_t) when grouped by level
+  // Reflection classes: reflecting an offset in x, y or z changes the M2L
+  // operator only by signs on the expansion components (x: (-1)^m conj,
+  // y: conj, z: (-1)^(n+m), on both the multipole columns and the local
+  // rows), so every group maps to a representative with non-negative offset
+  // components (canon) and the reflections that lead there (refl: bit d set
+  // when component d was negative).  Periodic groups (offsets carrying an
+  // image shift) are their own representatives.
+  int ncanon = 0;
+  std::vector<int> canon;          // per group
+  std::vector<uint8_t> refl;       // per group
+  std::vector<double> cdx;         // 3 per representative
+  std::vector<double> cscale;      // 2 per representative when grouped by level
 };
 // by_level: a group also fixes the source and target levels (the scaled FP32
 // operator depends on R_s and R_t); pairs whose radii differ from their
@@ -103,7 +115,8 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
 // Target-stationary layout of the grouped M2L pairs: targets are sorted by
 // (level, octant within the parent, cell) -- cells of one octant class share
 // their offsets -- and cut into tiles of TT targets.  A tile's pairs are
-// ordered by group and cut into slices of at most SW pairs of one group, so a
+// ordered by (representative, reflection) and cut into slices of at most SW
+// pairs of one group, so a
 // block can apply each group's operator to whole slices and keep the tile's
 // local expansions in LDS until one final store.  Tiles are listed in launch
 // order (dealt to the 8 XCDs in contiguous runs; empty tiles pad).
@@ -114,7 +127,8 @@ struct M2LTiles {
   int TT = 0, ntile = 0;
   std::vector<int> tgt;        // ntile*TT target cells (-1 pads), launch order
   std::vector<int> ptr;        // ntile+1 offsets into the slices
-  std::vector<int> slice;      // 4 per slice: group, first pair, count, 0
+  std::vector<int> slice;      // 4 per slice: representative, first pair, count, reflection bits
+                               // (the slice's pairs belong to one group)
   std::vector<int> src;        // source cell of every pair
   std::vector<uint8_t> lt;     // target of every pair as its index in the tile
   std::vector<int> item;       // 4 per work item: tile, first slice, end slice, split (0/1)

@@ -1,8 +1,10 @@
 // Host-side invariants of the target-stationary M2L layout (fmm::build_m2l_tiles
 // and its merge): every grouped pair appears exactly once, in the tile that
 // owns its target; a slice holds at most 16 pairs of one group with distinct
-// targets; a tile's targets are distinct and every target has one tile.
-// Prints "ok" or the first violation.
+// targets and names that group's representative and reflection; every
+// group's offset is its representative's reflected by its bits; a tile's
+// targets are distinct and every target has one tile.  Prints "ok" or the
+// first violation.
 #include <algorithm>
 #include <array>
 #include <cstdio>
@@ -16,6 +18,17 @@
 using namespace fmm;
 
 static int check(const Tree &t, const M2LGroups &G, const M2LTiles &T) {
+  if ((int)G.canon.size() != G.ngroup || (int)G.refl.size() != G.ngroup || (int)G.cdx.size() != 3 * G.ncanon)
+    return printf("reflection classes: bad sizes\n"), 1;
+  for (int g = 0; g < G.ngroup; g++) {
+    const int c = G.canon[g];
+    if (c < 0 || c >= G.ncanon) return printf("representative out of range\n"), 1;
+    for (int d = 0; d < 3; d++) {
+      const double want_d = ((G.refl[g] >> d) & 1) ? -G.cdx[3 * (size_t)c + d] : G.cdx[3 * (size_t)c + d];
+      if (G.dx[3 * (size_t)g + d] != want_d || G.cdx[3 * (size_t)c + d] < 0)
+        return printf("group %d is not its representative reflected\n", g), 1;
+    }
+  }
   std::map<std::pair<int, int>, int> want;  // (target, source) -> group
   for (int g = 0; g < G.ngroup; g++)
     for (int q = G.grp_ptr[g]; q < G.grp_ptr[g + 1]; q++) want[{G.tgt[q], G.src[q]}] = g;
@@ -29,15 +42,19 @@ static int check(const Tree &t, const M2LGroups &G, const M2LTiles &T) {
       if (!tiles_t.insert(c).second || !seen_t.insert(c).second) return printf("target %d twice\n", c), 1;
     }
     for (int s = T.ptr[b]; s < T.ptr[b + 1]; s++) {
-      const int g = T.slice[4 * s], p0 = T.slice[4 * s + 1], n = T.slice[4 * s + 2];
+      const int rep = T.slice[4 * s], p0 = T.slice[4 * s + 1], n = T.slice[4 * s + 2], rf = T.slice[4 * s + 3];
       if (n < 1 || n > 16) return printf("slice size %d\n", n), 1;
+      int g = -1;  // the slice's group, from its first pair
       std::set<int> lts;
       for (int k = p0; k < p0 + n; k++) {
         const int lt = T.lt[k];
         if (!lts.insert(lt).second) return printf("target repeated in a slice\n"), 1;
         const int c = T.tgt[(size_t)b * T.TT + lt];
         auto it = want.find({c, T.src[k]});
-        if (c < 0 || it == want.end() || it->second != g) return printf("pair not in its group\n"), 1;
+        if (c < 0 || it == want.end()) return printf("pair not grouped\n"), 1;
+        if (g < 0) g = it->second;
+        if (it->second != g) return printf("slice mixes groups\n"), 1;
+        if (G.canon[g] != rep || G.refl[g] != rf) return printf("slice names the wrong representative\n"), 1;
         want.erase(it);
         npairs++;
       }
