@@ -82,7 +82,7 @@ struct fmm_ctx {
   // M2M / L2L as per-level operators on MFMA (regular trees)
   bool xl = false;
   std::vector<int> xl_tptr;  // per child level: first L2L tile
-  DBuf xl_child8, xl_children, xl_tiles, xl_ops, xl_dx, xl_kinds;
+  DBuf xl_child8, xl_children, xl_tiles, xl_ops, xl_frag, xl_dx, xl_kinds;
   // periodic images: shift unit per image, far-image operator F (images >= 2)
   double3 cyc = {0.0, 0.0, 0.0};
   bool far = false;
@@ -696,6 +696,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     if ((rc = ensure(c, c->xl_ops, sizeof(double) * fmmdev::xlate_op_stride(c->p.P) * nop))) return rc;
     HIPCHK(fmmdev::launch_xlate_ops(c->p.P, c->xl_dx.as<double>(), c->xl_kinds.as<int>(), nop,
                                     c->xl_ops.as<double>(), 0, c->st));
+    if ((rc = ensure(c, c->xl_frag, sizeof(double) * fmmdev::xlate_frag_stride(c->p.P) * nop))) return rc;
+    HIPCHK(fmmdev::launch_ops_to_frag(c->p.P, c->xl_ops.as<double>(), nop, c->xl_frag.as<double>(), c->st));
   }
   if (c->use_tiles) {  // harmonics, then the dense operator, of every representative offset, once per plan
     const int nr = c->groups.ncanon;
@@ -820,12 +822,12 @@ int fmm_upward(fmm_ctx *c) {
   auto C = dev_cells(c);
   HIPCHK(fmmdev::launch_p2m(P, dev_bodies(c), C, c->leaves.as<int>(), (int)c->tree.leaves.size(),
                             c->M.as<double2>(), c->st));
-  const size_t os = fmmdev::xlate_op_stride(P);
+  const size_t os = fmmdev::xlate_frag_stride(P);
   for (int l = c->tree.depth; l >= 0; l--) {
     const int b = c->internal_ptr[l], e = c->internal_ptr[l + 1];
     if (e > b && c->xl)
       HIPCHK(fmmdev::launch_m2m_mfma(P, c->internal.as<int>() + b, c->xl_child8.as<int>() + 8 * (size_t)b, e - b,
-                                     c->xl_ops.as<double>() + os * 8 * l, c->M.as<double2>(), c->st));
+                                     c->xl_frag.as<double>() + os * 8 * l, c->M.as<double2>(), c->st));
     else if (e > b)
       HIPCHK(fmmdev::launch_m2m(P, C, c->internal.as<int>() + b, e - b, c->M.as<double2>(), c->st));
   }
@@ -846,7 +848,7 @@ int fmm_p2p(fmm_ctx *c) {
   if (c->timing) HIPCHK(hipEventRecord(c->kev[2], c->st));
   HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
       dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
-      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->st));
+      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->tree.adopted ? 1 : 0, c->st));
   if (c->timing) HIPCHK(hipEventRecord(c->kev[3], c->st));
   return FMM_OK;
 }
@@ -900,14 +902,14 @@ int fmm_downward(fmm_ctx *c) {
   if (c->tree.ncell == 0) return FMM_OK;
   const int P = c->p.P;
   auto C = dev_cells(c);
-  const size_t os = fmmdev::xlate_op_stride(P);
+  const size_t os = fmmdev::xlate_frag_stride(P);
   for (int l = 1; l <= c->tree.depth; l++) {
     const int b = c->level_ptr[l], e = c->level_ptr[l + 1];
     if (c->xl) {
       const int t0 = c->xl_tptr[l], t1 = c->xl_tptr[l + 1];
       if (t1 > t0)
         HIPCHK(fmmdev::launch_l2l_mfma(P, c->xl_tiles.as<int4>() + t0, t1 - t0, c->xl_children.as<int>(),
-                                       c->iparent.as<int>(), c->xl_ops.as<double>() + os * (8 * (c->tree.depth + 1) + 8 * l),
+                                       c->iparent.as<int>(), c->xl_frag.as<double>() + os * (8 * (c->tree.depth + 1) + 8 * l),
                                        c->L.as<double2>(), c->st));
     } else if (e > b) {
       HIPCHK(fmmdev::launch_l2l(P, C, c->level_cells.as<int>() + b, e - b, c->L.as<double2>(), c->st));

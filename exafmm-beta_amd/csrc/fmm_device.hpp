@@ -74,6 +74,10 @@ hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 
 // with dX = X_child - X_parent; xlate_op_stride(P) doubles each)
 bool xlate_mfma_supported(int P);
 int xlate_op_stride(int P);
+// the operators in MFMA A-operand order (what m2m_mfma / l2l_mfma read);
+// xlate_frag_stride(P) doubles each
+int xlate_frag_stride(int P);
+hipError_t launch_ops_to_frag(int P, const double *ops, int nop, double *frag, hipStream_t s);
 // accum != 0: sum of all operators into ops (row stride P^2, no padding)
 hipError_t launch_xlate_ops(int P, const double *dxs, const int *kinds, int nop, double *ops, int accum,
                             hipStream_t s);
@@ -96,9 +100,9 @@ hipError_t launch_m_to_f32(int P, const double2 *M, const double *R, int ncell, 
 hipError_t launch_m2l_mfma32(int P, const int4 *chunks, int nchunk, const double *gdx, const double2 *gscale,
                              const int *ptgt, const int *psrc, const float2 *M32, double2 *L, hipStream_t s);
 hipError_t launch_p2p32(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
-                        const int *src, double3 xper, double eps2, hipStream_t s);
+                        const int *src, double3 xper, double eps2, int mask_all, hipStream_t s);
 hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
-                      const int *src, double3 xper, double eps2, hipStream_t s);
+                      const int *src, double3 xper, double eps2, int mask_all, hipStream_t s);
 
 // Direct sum (include/traversal.h:355-377) of all n sources at nt targets
 // (SoA txyz); writes tpot/tax/tay/taz.

@@ -194,6 +194,7 @@ std::string adopt_tree(Tree &t, int n, int ncell, const int *iparent, const int 
   t = Tree();
   t.n = n;
   t.ncell = ncell;
+  t.adopted = true;
   t.perm.resize(n);
   for (int i = 0; i < n; i++) t.perm[i] = i;
   if (ncell == 0) {
@@ -937,6 +938,7 @@ void merge_plans(const std::vector<PlanPart> &parts, Tree &T, Lists &L, M2LGroup
   }
   T.n = (int)nb;
   T.ncell = (int)nc;
+  for (const PlanPart &p : parts) T.adopted = T.adopted || p.t->adopted;
   L.nm2l = npm;
   L.np2p = npp;
   L.m2l_ptr.assign(nc + 1, 0);

@@ -23,6 +23,9 @@ struct Tree {
   // Cells grouped by level (level_ptr[l] .. level_ptr[l+1] in level_cells)
   std::vector<int> level_ptr, level_cells;
   std::vector<int> leaves;   // leaf cells in cell order
+  // caller-built (adopt_tree): coincident bodies may sit in different leaves;
+  // in trees built here they always share a leaf
+  bool adopted = false;
 };
 
 // Interaction lists in CSR form per target cell, each list in the order the
