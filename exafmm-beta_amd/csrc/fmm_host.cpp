@@ -48,6 +48,18 @@ struct Builder {
   const double *x;
   int ncrit;
   std::vector<Node> pool;
+  // subtrees below `defer_level` are left as tasks (their roots made as
+  // placeholders) for other builders; -1: build everything here
+  struct Task {
+    int node;  // placeholder in this pool
+    int *bodies, *buffer;
+    int begin, end, level;
+    bool direction;
+    double X[3];
+  };
+  int defer_level = -1;
+  std::vector<Task> *tasks = nullptr;
+  int par_threads = 1;  // threads for the partition of large nodes
 
   int make(int begin, int end, const double *X) {
     Node nd;
@@ -78,17 +90,57 @@ struct Builder {
       if (direction) std::memcpy(buffer + begin, bodies + begin, sizeof(int) * (end - begin));
       return make(begin, end, X);
     }
+    if (level == defer_level) {
+      const int node = make(begin, end, X);
+      tasks->push_back({node, bodies, buffer, begin, end, level, direction, {X[0], X[1], X[2]}});
+      return node;
+    }
     int self = make(begin, end, X);
     int nb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = begin; i < end; i++) nb[octant(x + 3 * (size_t)bodies[i], X)]++;
-    int off[8], start[8];
-    for (int i = 0, o = begin; i < 8; i++) {
-      off[i] = start[i] = o;
-      o += nb[i];
-    }
-    for (int i = begin; i < end; i++) {
-      int b = bodies[i];
-      buffer[off[octant(x + 3 * (size_t)b, X)]++] = b;
+    int start[8];
+    if (par_threads > 1 && end - begin >= 65536) {
+      // stable partition by octant on host threads: thread k counts and then
+      // scatters its contiguous share after the shares of threads < k
+      const int T = par_threads;
+      std::vector<std::array<int, 8>> cnt(T), off(T);
+      auto run = [&](auto fn) {
+        std::vector<std::thread> th;
+        for (int k = 1; k < T; k++) th.emplace_back(fn, k);
+        fn(0);
+        for (auto &w : th) w.join();
+      };
+      run([&](int k) {
+        cnt[k].fill(0);
+        const int lo = begin + (int)((int64_t)(end - begin) * k / T), hi = begin + (int)((int64_t)(end - begin) * (k + 1) / T);
+        for (int i = lo; i < hi; i++) cnt[k][octant(x + 3 * (size_t)bodies[i], X)]++;
+      });
+      for (int k = 0; k < T; k++)
+        for (int o = 0; o < 8; o++) nb[o] += cnt[k][o];
+      for (int o = 0, a = begin; o < 8; o++) {
+        start[o] = a;
+        for (int k = 0; k < T; k++) {
+          off[k][o] = a;
+          a += cnt[k][o];
+        }
+      }
+      run([&](int k) {
+        const int lo = begin + (int)((int64_t)(end - begin) * k / T), hi = begin + (int)((int64_t)(end - begin) * (k + 1) / T);
+        for (int i = lo; i < hi; i++) {
+          const int b = bodies[i];
+          buffer[off[k][octant(x + 3 * (size_t)b, X)]++] = b;
+        }
+      });
+    } else {
+      for (int i = begin; i < end; i++) nb[octant(x + 3 * (size_t)bodies[i], X)]++;
+      int off[8];
+      for (int i = 0, o = begin; i < 8; i++) {
+        off[i] = start[i] = o;
+        o += nb[i];
+      }
+      for (int i = begin; i < end; i++) {
+        int b = bodies[i];
+        buffer[off[octant(x + 3 * (size_t)b, X)]++] = b;
+      }
     }
     double r = R0 / (1 << (level + 1));
     int kids[8];
@@ -164,7 +216,50 @@ Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const d
   for (int i = 0; i < n; i++) t.perm[i] = i;
   Builder B{x, ncrit, {}};
   B.pool.reserve(2 * (size_t)n / std::max(ncrit, 1) + 64);
+  const int nth = n >= 100000 ? host_threads() : 1;
+  std::vector<Builder::Task> tasks;
+  if (nth > 1) {  // levels 0-1 here (partitions threaded), the subtrees below on host threads
+    B.defer_level = 2;
+    B.tasks = &tasks;
+    B.par_threads = nth;
+  }
   int root = B.build(t.perm.data(), buffer.data(), 0, n, X, R, 0, false);
+  if (!tasks.empty()) {
+    const int ntop = (int)B.pool.size();
+    std::vector<Builder> sub(tasks.size(), Builder{x, ncrit, {}});
+    std::vector<int> sroot(tasks.size());
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < tasks.size();) {
+        const Builder::Task &tk = tasks[k];
+        sroot[k] = sub[k].build(tk.bodies, tk.buffer, tk.begin, tk.end, tk.X, R, tk.level, tk.direction);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < nth; i++) th.emplace_back(work);
+    work();
+    for (auto &w : th) w.join();
+    // stitch: each subtree's nodes appended, its root copied onto the placeholder
+    for (size_t k = 0; k < tasks.size(); k++) {
+      const int off = (int)B.pool.size();
+      for (Node nd : sub[k].pool) {
+        for (int i = 0; i < 8; i++)
+          if (nd.child[i] >= 0) nd.child[i] += off;
+        B.pool.push_back(nd);
+      }
+      B.pool[tasks[k].node] = B.pool[off + sroot[k]];
+    }
+    // node counts of the top levels: children were made after their parents
+    for (int i = ntop - 1; i >= 0; i--) {
+      bool placeholder = false;
+      for (const Builder::Task &tk : tasks) placeholder = placeholder || tk.node == i;
+      if (placeholder) continue;
+      int nnode = 1;
+      for (int c = 0; c < 8; c++)
+        if (B.pool[i].child[c] >= 0) nnode += B.pool[B.pool[i].child[c]].nnode;
+      B.pool[i].nnode = nnode;
+    }
+  }
   int nc = B.pool[root].nnode;
   t.ncell = nc;
   t.iparent.resize(nc);
@@ -604,7 +699,7 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     int e, img;
     double rs, rt;
   };
-  std::vector<int> lid(np);  // run-local group of each pair, -1: not grouped
+  bigvec<int> lid(np);  // run-local group of each pair, -1: not grouped
   std::vector<std::vector<Fresh>> fresh(T);
   auto par = [&](auto fn) {
     std::vector<std::thread> th;
@@ -876,51 +971,99 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
   std::vector<int64_t> pcount(nt + 1, 0);
   for (int64_t q = 0; q < np; q++) pcount[tile_of[G.tgt[q]] + 1]++;
   for (int i = 0; i < nt; i++) pcount[i + 1] += pcount[i];
-  std::vector<int> pgrp(np), psrc(np);
-  std::vector<uint8_t> plt(np);
+  bigvec<int> pgrp(np), psrc(np);
+  bigvec<uint8_t> plt(np);
   std::vector<int> gorder(G.ngroup);
   for (int g = 0; g < G.ngroup; g++) gorder[g] = g;
   std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
     return G.canon[a] != G.canon[b] ? G.canon[a] < G.canon[b] : G.refl[a] < G.refl[b];
   });
+  // runs of consecutive groups (in that order) with about equal pair counts,
+  // one per thread; run r fills its slots of every tile after runs < r, so the
+  // result is the single-threaded order
+  const int R = np >= 200000 ? host_threads() : 1;
+  std::vector<int> gcut(R + 1, G.ngroup);
   {
-    std::vector<int64_t> fill(pcount.begin(), pcount.end() - 1);
-    for (int g : gorder)
+    gcut[0] = 0;
+    int64_t acc = 0;
+    int r = 1;
+    for (int k = 0; k < G.ngroup && r < R; k++) {
+      const int g = gorder[k];
+      acc += G.grp_ptr[g + 1] - G.grp_ptr[g];
+      while (r < R && acc >= np * r / R) gcut[r++] = k + 1;
+    }
+  }
+  auto par = [&](int nthr, auto fn) {
+    std::vector<std::thread> th;
+    for (int r = 1; r < nthr; r++) th.emplace_back(fn, r);
+    fn(0);
+    for (auto &x : th) x.join();
+  };
+  std::vector<std::vector<int64_t>> off(R, std::vector<int64_t>(nt, 0));
+  par(R, [&](int r) {
+    for (int k = gcut[r]; k < gcut[r + 1]; k++) {
+      const int g = gorder[k];
+      for (int q = G.grp_ptr[g]; q < G.grp_ptr[g + 1]; q++) off[r][tile_of[G.tgt[q]]]++;
+    }
+  });
+  for (int b = 0; b < nt; b++) {
+    int64_t a = pcount[b];
+    for (int r = 0; r < R; r++) {
+      const int64_t k = off[r][b];
+      off[r][b] = a;
+      a += k;
+    }
+  }
+  par(R, [&](int r) {
+    std::vector<int64_t> &fill = off[r];
+    for (int k = gcut[r]; k < gcut[r + 1]; k++) {
+      const int g = gorder[k];
       for (int q = G.grp_ptr[g]; q < G.grp_ptr[g + 1]; q++) {
         const int c = G.tgt[q];
-        const int64_t k = fill[tile_of[c]]++;
-        pgrp[k] = g;
-        psrc[k] = G.src[q];
-        plt[k] = (uint8_t)local[c];
-      }
-  }
-  // tiles in key order (neighbours adjacent); the work items built from them
-  // are dealt to the XCDs by split_m2l_tiles
-  T.ntile = nt;
-  T.tgt.assign((size_t)T.ntile * TT, -1);
-  T.src.reserve(np);
-  T.lt.reserve(np);
-  for (int b = 0; b < T.ntile; b++) {
-    const int tile = b;
-    {
-      for (int j = 0; j < TT; j++) {
-        const size_t i = (size_t)tile * TT + j;
-        if (i < ord.size()) T.tgt[(size_t)b * TT + j] = tg[ord[i]];
-      }
-      for (int64_t k = pcount[tile]; k < pcount[tile + 1];) {
-        const int g = pgrp[k];
-        int64_t e = k;
-        while (e < pcount[tile + 1] && pgrp[e] == g && e - k < SW) e++;
-        T.slice.insert(T.slice.end(), {G.canon[g], (int)T.src.size(), (int)(e - k), (int)G.refl[g]});
-        for (int64_t u = k; u < e; u++) {
-          T.src.push_back(psrc[u]);
-          T.lt.push_back(plt[u]);
-        }
-        k = e;
+        const int64_t kk = fill[tile_of[c]]++;
+        pgrp[kk] = g;
+        psrc[kk] = G.src[q];
+        plt[kk] = (uint8_t)local[c];
       }
     }
-    T.ptr.push_back((int)(T.slice.size() / 4));
-  }
+  });
+  // tiles in key order (neighbours adjacent); the work items built from them
+  // are dealt to the XCDs by split_m2l_tiles.  Slices: runs of at most SW
+  // pairs of one group inside a tile, pairs kept in place.
+  T.ntile = nt;
+  T.tgt.assign((size_t)T.ntile * TT, -1);
+  for (size_t i = 0; i < ord.size(); i++) T.tgt[i] = tg[ord[i]];
+  auto slices_of = [&](int b, int *out) {  // count (out == nullptr) or write the slices of tile b
+    int ns = 0;
+    for (int64_t k = pcount[b]; k < pcount[b + 1];) {
+      const int g = pgrp[k];
+      int64_t e = k;
+      while (e < pcount[b + 1] && pgrp[e] == g && e - k < SW) e++;
+      if (out) {
+        int *o = out + 4 * (size_t)ns;
+        o[0] = G.canon[g];
+        o[1] = (int)k;
+        o[2] = (int)(e - k);
+        o[3] = (int)G.refl[g];
+      }
+      ns++;
+      k = e;
+    }
+    return ns;
+  };
+  T.ptr.assign(nt + 1, 0);
+  const int RT = std::min(R, std::max(nt, 1));
+  par(RT, [&](int r) {
+    for (int b = (int)((int64_t)nt * r / RT); b < (int)((int64_t)nt * (r + 1) / RT); b++) T.ptr[b + 1] = slices_of(b, nullptr);
+  });
+  for (int b = 0; b < nt; b++) T.ptr[b + 1] += T.ptr[b];
+  T.slice.resize(4 * (size_t)T.ptr[nt]);
+  par(RT, [&](int r) {
+    for (int b = (int)((int64_t)nt * r / RT); b < (int)((int64_t)nt * (r + 1) / RT); b++)
+      slices_of(b, T.slice.data() + 4 * (size_t)T.ptr[b]);
+  });
+  T.src = std::move(psrc);
+  T.lt = std::move(plt);
   split_m2l_tiles(T, seg);
   return T;
 }

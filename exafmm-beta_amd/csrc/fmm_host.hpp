@@ -4,10 +4,24 @@
 // compared with the reference FMM at the same P, theta and ncrit.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace fmm {
+
+// Allocator whose resize() leaves new elements uninitialised: the large pair
+// arrays below are filled right after by parallel passes, and zero-filling
+// them first (one thread, fresh pages) costs as much as building them.
+template <class T> struct NoInit : std::allocator<T> {
+  template <class U> struct rebind { using other = NoInit<U>; };
+  NoInit() = default;
+  template <class U> NoInit(const NoInit<U> &) noexcept {}
+  template <class U> void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
+  template <class U, class... A> void construct(U *p, A &&...a) { ::new (static_cast<void *>(p)) U(std::forward<A>(a)...); }
+};
+template <class T> using bigvec = std::vector<T, NoInit<T>>;
 
 // Flattened octree in the reference's cell order (nodes2cells,
 // include/build_tree.h:123-165): children of a cell are contiguous, bodies of
@@ -90,7 +104,7 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3],
 // Pairs whose key does not fit the packing go to the per-target lists rest_*.
 struct M2LGroups {
   int ngroup = 0;
-  std::vector<int> tgt, src;       // grouped pairs
+  bigvec<int> tgt, src;            // grouped pairs
   std::vector<int> grp_ptr;        // ngroup+1
   std::vector<double> dx;          // 3 per group: representative offset
   std::vector<int> chunk;          // 4 per chunk
@@ -132,8 +146,8 @@ struct M2LTiles {
   std::vector<int> ptr;        // ntile+1 offsets into the slices
   std::vector<int> slice;      // 4 per slice: representative, first pair, count, reflection bits
                                // (the slice's pairs belong to one group)
-  std::vector<int> src;        // source cell of every pair
-  std::vector<uint8_t> lt;     // target of every pair as its index in the tile
+  bigvec<int> src;             // source cell of every pair
+  bigvec<uint8_t> lt;          // target of every pair as its index in the tile
   std::vector<int> item;       // 4 per work item: tile, first slice, end slice, split (0/1)
 };
 M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int seg = 64);
