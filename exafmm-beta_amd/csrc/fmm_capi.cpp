@@ -449,6 +449,19 @@ int fmm_plan_tree(const fmm_plan *pl, int *perm, int *iparent, int *ichild, int 
   return FMM_OK;
 }
 
+int fmm_plan_list_counts(const fmm_plan *pl, uint64_t *icell, int *nump2p, int *numm2l) {
+  if (!pl) return FMM_ERR_ARG;
+  fmm::list_counts(pl->tree, pl->lists, icell, nump2p, numm2l);
+  return FMM_OK;
+}
+
+int fmm_get_list_counts(fmm_ctx *c, uint64_t *icell, int *nump2p, int *numm2l) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies");
+  fmm::list_counts(c->tree, c->lists, icell, nump2p, numm2l);
+  return FMM_OK;
+}
+
 int fmm_plan_lists(const fmm_plan *pl, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
   if (!pl) return FMM_ERR_ARG;
   copy_lists(pl->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);

@@ -883,6 +883,29 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   return G;
 }
 
+void list_counts(const Tree &t, const Lists &L, uint64_t *icell, int *nump2p, int *numm2l) {
+  const int nc = t.ncell;
+  for (int c = 0; c < nc; c++) {
+    if (nump2p) nump2p[c] = L.p2p_ptr[c + 1] - L.p2p_ptr[c];
+    if (numm2l) numm2l[c] = L.m2l_ptr[c + 1] - L.m2l_ptr[c];
+  }
+  if (!icell || nc == 0) return;
+  double xmin[3];
+  for (int d = 0; d < 3; d++) xmin[d] = t.X[d] - t.R[0];
+  for (int c = 0; c < nc; c++) {
+    const int level = t.level[c];
+    const double diameter = 2 * t.R[c];
+    int iX[3];
+    for (int d = 0; d < 3; d++) iX[d] = (int)((t.X[3 * (size_t)c + d] - xmin[d]) / diameter);
+    uint64_t index = (((uint64_t)1 << (3 * level)) - 1) / 7;
+    for (int l = 0; l < level; l++) {
+      for (int d = 0; d < 3; d++) index += (uint64_t)(iX[d] & 1) << (3 * l + d);
+      for (int d = 0; d < 3; d++) iX[d] >>= 1;
+    }
+    icell[c] = index;
+  }
+}
+
 void deal_m2l_chunks(M2LGroups &G, int chunk_pairs) {
   const int ng = G.ngroup;
   // Chunks sorted by their first target (cell index ranges are subtrees, so

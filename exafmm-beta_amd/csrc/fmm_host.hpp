@@ -129,6 +129,12 @@ struct M2LGroups {
 // group's are left ungrouped.
 M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool by_level = false,
                            const double *cycle = nullptr);
+// Traversal::countList / writeList (include/traversal.h:30-38, 412-421): per
+// cell its Morton key (BuildTree::getKey, include/build_tree.h:111-120, with
+// 64-bit arithmetic at every level) and the lengths of its P2P and M2L lists.
+// Any output may be null.
+void list_counts(const Tree &t, const Lists &L, uint64_t *icell, int *nump2p, int *numm2l);
+
 // Target-stationary layout of the grouped M2L pairs: targets are sorted by
 // (level, octant within the parent, cell) -- cells of one octant class share
 // their offsets -- and cut into tiles of TT targets.  A tile's pairs are

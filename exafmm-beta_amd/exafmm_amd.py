@@ -28,7 +28,7 @@ SYMBOLS = [
     "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
     "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
-    "fmm_dipole_correction", "fmm_md_evaluate",
+    "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts",
 ]
 
 FMM_FP64 = 0
@@ -133,6 +133,9 @@ def load():
         L.fmm_make_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         L.fmm_direct.argtypes = [vp, ctypes.c_int, _dp, _dp]
         L.fmm_md_evaluate.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        _up = ctypes.POINTER(ctypes.c_uint64)
+        L.fmm_plan_list_counts.argtypes = [vp, _up, _ip, _ip]
+        L.fmm_get_list_counts.argtypes = [vp, _up, _ip, _ip]
         L.fmm_set_tree.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
                                    _dp, _dp]
         L.fmm_plan_from_tree.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
@@ -337,6 +340,15 @@ class FMM:
                                          _ptr(p2p_ptr, _ip), _ptr(p2p_src, _ip)), "fmm_get_lists")
         return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]
 
+    def list_counts(self):
+        """Traversal::countList per cell: (ICELL Morton keys, numP2P, numM2L)."""
+        nc = self.stats()["ncell"]
+        key = np.zeros(max(nc, 1), np.uint64)
+        p2p, m2l = np.zeros(max(nc, 1), np.int32), np.zeros(max(nc, 1), np.int32)
+        self._check(load().fmm_get_list_counts(self._h, key.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                               _ptr(p2p, _ip), _ptr(m2l, _ip)), "fmm_get_list_counts")
+        return key[:nc], p2p[:nc], m2l[:nc]
+
 
 class Plan:
     """Host-only tree + interaction lists (fmm_plan_*): no device needed."""
@@ -430,6 +442,15 @@ class Plan:
         load().fmm_plan_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip), _ptr(p2p_ptr, _ip),
                               _ptr(p2p_src, _ip))
         return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]
+
+    def list_counts(self):
+        """Traversal::countList per cell: (ICELL Morton keys, numP2P, numM2L)."""
+        nc = self.stats()["ncell"]
+        key = np.zeros(max(nc, 1), np.uint64)
+        p2p, m2l = np.zeros(max(nc, 1), np.int32), np.zeros(max(nc, 1), np.int32)
+        load().fmm_plan_list_counts(self._h, key.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                    _ptr(p2p, _ip), _ptr(m2l, _ip))
+        return key[:nc], p2p[:nc], m2l[:nc]
 
 
 class Server:

@@ -48,6 +48,9 @@ int main(int argc, char **argv) {
       upDownPass.downwardPass(cells);
       logger::printTitle("Total runtime");
       logger::stopTimer("Total FMM");
+      // examples/fmm.cxx:88 (a list file only with --path, as the reference
+      // writes one only when built with list counting)
+      if (std::string(args.path) != "./") traversal.writeList(cells, 0);
 
       const int numTargets = 100;
       buffer = bodies;

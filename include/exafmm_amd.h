@@ -106,6 +106,14 @@ int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q);
 typedef struct fmm_plan fmm_plan;
 int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn);
 void fmm_plan_destroy(fmm_plan *pl);
+/* Traversal::countList / writeList (include/traversal.h:30-38, 412-421):
+ * per cell (ncell entries each) its Morton key ICELL (BuildTree::getKey,
+ * include/build_tree.h:111-120, 132, computed in 64 bits at every level; the
+ * reference's int shifts overflow past level 10) and the lengths of its P2P
+ * and M2L lists (periodic image pairs included).  Any array may be NULL. */
+int fmm_plan_list_counts(const fmm_plan *pl, uint64_t *icell, int *nump2p, int *numm2l);
+int fmm_get_list_counts(fmm_ctx *c, uint64_t *icell, int *nump2p, int *numm2l);
+
 /* fills n, ncell, nleaf, depth, pair counts, bounds and host times */
 int fmm_plan_stats(const fmm_plan *pl, fmm_stats *s);
 int fmm_plan_tree(const fmm_plan *pl, int *perm, int *iparent, int *ichild, int *nchild,

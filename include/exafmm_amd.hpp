@@ -25,9 +25,11 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <map>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -76,6 +78,7 @@ typedef Bodies::iterator B_iter;
 //! include/types.h:98-119
 struct CellBase {
   int IPARENT, ICHILD, NCHILD, IBODY, NBODY;
+  int numP2P = 0, numM2L = 0;  // interaction list sizes (include/types.h:104-107)
   uint64_t ICELL;
   real_t WEIGHT;
   vec3 X;
@@ -168,6 +171,8 @@ class BuildTree {
     std::vector<int> perm(n), ip(nc), ic(nc), nch(nc), ib(nc), nb(nc);
     std::vector<double> X(3 * (size_t)nc), R(nc);
     fmm_plan_tree(pl, perm.data(), ip.data(), ic.data(), nch.data(), ib.data(), nb.data(), X.data(), R.data());
+    std::vector<uint64_t> key(nc);
+    fmm_plan_list_counts(pl, key.data(), nullptr, nullptr);
     fmm_plan_destroy(pl);
     buffer.resize(n);
     for (int i = 0; i < n; i++) buffer[i] = bodies[perm[i]];
@@ -181,7 +186,7 @@ class BuildTree {
       C.NCHILD = nch[c];
       C.IBODY = ib[c];
       C.NBODY = nb[c];
-      C.ICELL = 0;
+      C.ICELL = key[c];  // Morton key (include/build_tree.h:132)
       C.WEIGHT = 0;
       for (int d = 0; d < 3; d++) C.X[d] = X[3 * (size_t)c + d];
       C.R = R[c];
@@ -250,6 +255,12 @@ class Kernel {
       ctx = nullptr;
       throw Error(m);
     }
+  }
+  //! drop the device context (remade by the next pass)
+  void release() {
+    if (ctx) fmm_destroy(ctx);
+    ctx = nullptr;
+    uploaded = nullptr;
   }
   void check(int rc, const char *what) const {
     if (rc != FMM_OK) throw Error(std::string(what) + ": " + fmm_last_error(ctx));
@@ -349,11 +360,16 @@ class Traversal {
   const real_t theta;
   const int nspawn;
   const int images;
+  const std::string path;
 
  public:
-  Traversal(Kernel &_kernel, real_t _theta, int _nspawn, int _images, const char * = "./")
-      : kernel(_kernel), theta(_theta), nspawn(_nspawn), images(_images) {
-    kernel.configure(theta, nspawn, images, kernel.cycle);
+  Traversal(Kernel &_kernel, real_t _theta, int _nspawn, int _images, const char *_path = "./")
+      : kernel(_kernel), theta(_theta), nspawn(_nspawn), images(_images), path(_path ? _path : "./") {
+    // the device context is made on first use with these parameters
+    kernel.theta = theta;
+    kernel.nspawn = nspawn;
+    kernel.images = images;
+    kernel.release();
   }
   //! images > 0 and a period other than the context's: remake the context
   //! (the lists depend on the period) and, if a tree was up, re-run its upward pass
@@ -368,7 +384,10 @@ class Traversal {
       kernel.check(fmm_upward(kernel.ctx), "fmm_upward");
     }
   }
-  void initListCount(Cells &) {}
+  //! include/traversal.h:298-306
+  void initListCount(Cells &cells) {
+    for (Cell &C : cells) C.numP2P = C.numM2L = 0;
+  }
   void initWeight(Cells &) {}
   void traverse(Cells &icells, Cells &jcells, vec3 cycle, bool, real_t = 1) {
     if (icells.empty() || jcells.empty()) return;
@@ -379,6 +398,13 @@ class Traversal {
     kernel.check(fmm_p2p(kernel.ctx), "fmm_p2p");
     kernel.check(fmm_m2l(kernel.ctx), "fmm_m2l");
     kernel.check(fmm_synchronize(kernel.ctx), "fmm_synchronize");
+    // countList (include/traversal.h:30-38): the list sizes of every target cell
+    std::vector<int> np2p(icells.size()), nm2l(icells.size());
+    kernel.check(fmm_get_list_counts(kernel.ctx, nullptr, np2p.data(), nm2l.data()), "fmm_get_list_counts");
+    for (size_t c = 0; c < icells.size(); c++) {
+      icells[c].numP2P += np2p[c];
+      icells[c].numM2L += nm2l[c];
+    }
     logger::stopTimer("Traverse");
   }
   //! direct sum of jbodies (and, with images > 0, of their copies shifted by
@@ -415,7 +441,15 @@ class Traversal {
               << std::setw(logger::stringLength) << std::left << "M2L calls"
               << " : " << s.m2l_pairs << std::endl;
   }
-  void writeList(Cells, int) {}
+  //! include/traversal.h:412-421: "ICELL numP2P numM2L" per cell into <path>list<rank, 6 digits>.dat
+  void writeList(Cells cells, int mpirank) {
+    std::ostringstream name;
+    name << path << "list" << std::setfill('0') << std::setw(6) << mpirank << ".dat";
+    std::ofstream f(name.str().c_str());
+    for (const Cell &C : cells)
+      f << std::setw(logger::stringLength) << std::left << C.ICELL << " " << C.numP2P << " " << C.numM2L
+        << std::endl;
+  }
 };
 
 //! include/dataset.h: seeded drand48 generators in the reference's draw order

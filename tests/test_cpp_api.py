@@ -43,7 +43,7 @@ def _run_dump(exe, *args):
     lo_hi = np.array(out[1].split(), float)
     nb, nc = map(int, out[2].split())
     bodies = np.array([l.split() for l in out[3:3 + nb]], float).reshape(nb, 5)
-    cells = np.array([l.split() for l in out[3 + nb:3 + nb + nc]], float).reshape(nc, 10)
+    cells = np.array([l.split()[:10] for l in out[3 + nb:3 + nb + nc]], float).reshape(nc, 10)
     return head, lo_hi, bodies, cells
 
 
@@ -149,6 +149,23 @@ def test_example_driver_periodic(images):
 
 
 @pytest.mark.gpu
+def test_example_driver_writes_list_counts(tmp_path):
+    """examples/fmm.cxx's writeList after traverse: per cell the Morton key
+    and the sizes of its P2P and M2L lists, equal to the oracle's."""
+    exe = os.path.join(PKG, "lib", "fmm")
+    path = str(tmp_path) + "/"
+    out = subprocess.run([exe, "-n", "4096", "-p", path], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = np.array([l.split() for l in open(path + "list000000.dat").read().strip().split("\n")], np.uint64)
+    x = O.data_cube(4096, 0)
+    t = O.Tree(x, 64)
+    ci, cj, ty = t.traverse(0.4, 5000)
+    assert np.array_equal(rows[:, 0], t.icell)
+    assert np.array_equal(rows[:, 1].astype(int), np.bincount(ci[ty == 1], minlength=t.ncell))
+    assert np.array_equal(rows[:, 2].astype(int), np.bincount(ci[ty == 0], minlength=t.ncell))
+
+
+@pytest.mark.gpu
 def test_set_tree_equals_set_bodies():
     F = _F()
     x = O.data_plummer(6000, 4)
@@ -215,3 +232,47 @@ def test_plan_from_tree_rejects_inconsistent_trees():
         with pytest.raises(F.FMMError, match="fmm_plan_from_tree"):
             F.Plan(xs, tree=_tree_dict(t, **bad))
     F.Plan(xs, tree=_tree_dict(t))      # the unmodified tree is accepted
+
+
+def test_cpp_morton_keys_and_write_list(tree_dump, tmp_path):
+    """BuildTree's ICELL (BuildTree::getKey, include/build_tree.h:111-120,132)
+    equals the oracle's; Traversal::writeList writes "ICELL numP2P numM2L" per
+    cell to <path>list<rank>.dat (include/traversal.h:412-421)."""
+    path = str(tmp_path) + "/"
+    head, lo_hi, b, c = _run_dump(tree_dump, "-n", "3000", "-c", "16", "-p", path)
+    out = subprocess.run([tree_dump, "-n", "3000", "-c", "16", "-p", path], check=True, capture_output=True,
+                         text=True).stdout.split("\n")
+    nb, nc = map(int, out[2].split())
+    keys = np.array([int(l.split()[10]) for l in out[3 + nb:3 + nb + nc]], np.uint64)
+    x = O.data_cube(3000, 0)
+    t = O.Tree(x, 16)
+    assert np.array_equal(keys, t.icell)
+    rows = [l.split() for l in open(path + "list000003.dat").read().strip().split("\n")]
+    assert len(rows) == nc
+    for i, r in enumerate(rows):
+        assert (int(r[0]), int(r[1]), int(r[2])) == (int(keys[i]), i % 7, i % 11)
+
+
+def test_plan_list_counts_match_oracle():
+    """fmm_plan_list_counts: numP2P / numM2L of every cell are the oracle
+    traversal's per-target pair counts; the keys nest (a child's key is its
+    parent's shifted by 3 bits plus its octant, above the level offset)."""
+    F = _F()
+    x = O.data_plummer(20000, 3)
+    pl = F.Plan(x, ncrit=32, theta=0.4, nspawn=1000)
+    key, np2p, nm2l = pl.list_counts()
+    t = O.Tree(x, 32)
+    ci, cj, ty = t.traverse(0.4, 1000)
+    assert np.array_equal(np2p, np.bincount(ci[ty == 1], minlength=t.ncell))
+    assert np.array_equal(nm2l, np.bincount(ci[ty == 0], minlength=t.ncell))
+    assert np.array_equal(key, t.icell)
+    tr = pl.tree()
+    off = lambda l: ((1 << (3 * l)) - 1) // 7
+    level = np.zeros(t.ncell, int)
+    for cc in range(1, t.ncell):
+        level[cc] = level[tr["iparent"][cc]] + 1
+    for cc in range(1, t.ncell):
+        p, lc = tr["iparent"][cc], level[cc]
+        oct_ = sum(1 << d for d in range(3) if tr["X"][cc][d] > tr["X"][p][d])
+        assert (int(key[cc]) - off(lc)) >> 3 == int(key[p]) - off(lc - 1)
+        assert (int(key[cc]) - off(lc)) & 7 == oct_
