@@ -58,6 +58,10 @@ struct fmm_ctx {
   hipStream_t st = nullptr;
   hipEvent_t ev[5] = {};
   hipEvent_t kev[4] = {};  // M2L kernel begin/end, P2P kernel begin/end
+  // fmm_evaluate runs P2P on a second stream (default priority), beside the
+  // upward pass and M2L: fork after the previous work, join before L2P
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   bool timing = false;     // set by fmm_evaluate
   fmm::Tree tree;
   fmm::Lists lists;
@@ -245,8 +249,11 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
         }
       }
       HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
       for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
       for (auto &e : c->kev) HIPCHK(hipEventCreate(&e));
+      HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
       return FMM_OK;
     }();
   }
@@ -262,6 +269,10 @@ void fmm_destroy(fmm_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
     if (e) (void)hipEventDestroy(e);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;  // the device buffers are freed here, with the device current
 }
@@ -847,23 +858,30 @@ int fmm_upward(fmm_ctx *c) {
   return FMM_OK;
 }
 
+namespace {
+// P2P (targets overwritten) on stream s
+int p2p_on(fmm_ctx *c, hipStream_t s) {
+  const size_t bytes = sizeof(double) * c->n;
+  if (c->n > 0) {
+    HIPCHK(hipMemsetAsync(c->pot.p, 0, bytes, s));
+    HIPCHK(hipMemsetAsync(c->ax.p, 0, bytes, s));
+    HIPCHK(hipMemsetAsync(c->ay.p, 0, bytes, s));
+    HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, s));
+  }
+  const double3 xper = c->cyc;  // image shifts of the source codes
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[2], s));
+  HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
+      dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
+      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->tree.adopted ? 1 : 0, s));
+  if (c->timing) HIPCHK(hipEventRecord(c->kev[3], s));
+  return FMM_OK;
+}
+}  // namespace
+
 int fmm_p2p(fmm_ctx *c) {
   int rc = check_ready(c);
   if (rc) return rc;
-  const size_t bytes = sizeof(double) * c->n;
-  if (c->n > 0) {
-    HIPCHK(hipMemsetAsync(c->pot.p, 0, bytes, c->st));
-    HIPCHK(hipMemsetAsync(c->ax.p, 0, bytes, c->st));
-    HIPCHK(hipMemsetAsync(c->ay.p, 0, bytes, c->st));
-    HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, c->st));
-  }
-  const double3 xper = c->cyc;  // image shifts of the source codes
-  if (c->timing) HIPCHK(hipEventRecord(c->kev[2], c->st));
-  HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
-      dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
-      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->tree.adopted ? 1 : 0, c->st));
-  if (c->timing) HIPCHK(hipEventRecord(c->kev[3], c->st));
-  return FMM_OK;
+  return p2p_on(c, c->st);
 }
 
 int fmm_m2l(fmm_ctx *c) {
@@ -938,13 +956,22 @@ int fmm_evaluate(fmm_ctx *c) {
   if (rc) return rc;
   c->timing = true;
   rc = [&]() -> int {
+  // P2P needs the bodies only: it runs on the second stream beside the
+  // upward pass and M2L (vector ALUs beside the matrix cores), and L2P, which
+  // adds into the same targets, waits for it
+  // (forking after the upward pass instead, so that P2P runs beside M2L, was
+  // measured no faster: M2L's LDS leaves no room for P2P blocks on a CU)
   HIPCHK(hipEventRecord(c->ev[0], c->st));
+  HIPCHK(hipEventRecord(c->fork, c->st));
+  HIPCHK(hipStreamWaitEvent(c->st2, c->fork, 0));
+  if ((rc = p2p_on(c, c->st2))) return rc;
+  HIPCHK(hipEventRecord(c->join, c->st2));
   if ((rc = fmm_upward(c))) return rc;
   HIPCHK(hipEventRecord(c->ev[1], c->st));
-  if ((rc = fmm_p2p(c))) return rc;
   HIPCHK(hipEventRecord(c->ev[2], c->st));
   if ((rc = fmm_m2l(c))) return rc;
   HIPCHK(hipEventRecord(c->ev[3], c->st));
+  HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   if ((rc = fmm_downward(c))) return rc;
   HIPCHK(hipEventRecord(c->ev[4], c->st));
   HIPCHK(hipEventSynchronize(c->ev[4]));
@@ -953,9 +980,8 @@ int fmm_evaluate(fmm_ctx *c) {
   float tot;
   HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[4]));
   c->stats.dev_upward_ms = ms[0];
-  c->stats.dev_p2p_ms = ms[1];
   c->stats.dev_m2l_ms = ms[2];
-  c->stats.dev_downward_ms = ms[3];
+  c->stats.dev_downward_ms = ms[3];  // includes any wait for P2P
   c->stats.dev_total_ms = tot;
   float km = 0.f, kp = 0.f;
   if (c->tree.ncell > 0) {
@@ -964,6 +990,7 @@ int fmm_evaluate(fmm_ctx *c) {
   }
   c->stats.dev_m2l_kernel_ms = km;
   c->stats.dev_p2p_kernel_ms = kp;
+  c->stats.dev_p2p_ms = kp;  // beside the upward pass and M2L
   return FMM_OK;
   }();
   c->timing = false;
