@@ -134,6 +134,28 @@ def md_bench(F, name, nreq, log):
                out[prec]["host_plan_ms_mean"], out[prec]["latency_ms"]["p99"]))
     p64, f64 = out["fp64"].pop("_first")
     p32, f32 = out["mixed"].pop("_first")
+    # validation of the first step against the Ewald sum (device), alpha and
+    # ksize chosen so that the real and reciprocal truncations are below 1e-7:
+    # erfc(alpha * cutoff) < 1e-7 with cutoff = L/2, exp(-K^2 / 4 alpha^2) < 1e-7
+    Lb = w["box"]
+    alpha = 3.9 / (Lb / 2)
+    ksize = int(math.ceil(2 * alpha * math.sqrt(math.log(1e7)) * Lb / (2 * math.pi)))
+    f = F.FMM(P=w["P"], images=w["images"], cycle=Lb)
+    t0 = time.perf_counter()
+    pe, fe = f.ewald(x0, q, ksize, alpha, 0.25 / math.pi, Lb / 2)
+    t_ew = time.perf_counter() - t0
+    f.close()
+    ee = float(q @ pe)
+    out["ewald_first_step"] = {
+        "ksize": ksize, "alpha": alpha, "cutoff": Lb / 2, "seconds": t_ew, "energy": ee,
+        "fp64": {"energy_rel": abs(q @ p64 - ee) / abs(ee),
+                 "acc_rel_l2": float(np.linalg.norm(f64 - fe) / np.linalg.norm(fe))},
+        "mixed": {"energy_rel": abs(q @ p32 - ee) / abs(ee),
+                  "acc_rel_l2": float(np.linalg.norm(f32 - fe) / np.linalg.norm(fe))},
+        "note": "the reference's periodic FMM leaves the image shell at distance 2 out at theta <= 0.5"}
+    log("[md] Ewald (ksize %d, alpha %.4f, %.2f s): FMM fp64 energy %.2e acc %.2e"
+        % (ksize, alpha, t_ew, out["ewald_first_step"]["fp64"]["energy_rel"],
+           out["ewald_first_step"]["fp64"]["acc_rel_l2"]))
     out["mixed_vs_fp64_first_step"] = {
         "energy_rel": abs(q @ p32 - q @ p64) / abs(q @ p64),
         "acc_rel_l2": float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))}

@@ -95,6 +95,7 @@ struct fmm_ctx {
   bool mixed = false;
   DBuf g_sc, cr, M32;
   DBuf dip;  // dipole partial sums
+  DBuf ew_b, ew_t, ew_k, ew_s;  // fmm_ewald: bodies, targets, wave vectors, structure factors
 };
 
 struct fmm_plan {
@@ -1074,6 +1075,56 @@ int fmm_md_evaluate(fmm_ctx *c, int n, const double *x, const double *q, double 
   for (size_t i = 0; i < (size_t)n; i++) {  // TRG starts from the caller's p, f (md_distributed.cxx:209-212)
     p[i] += trg[4 * i];
     for (int d = 0; d < 3; d++) f[3 * i + d] += trg[4 * i + 1 + d];
+  }
+  return FMM_OK;
+}
+
+int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, double alpha, double sigma,
+              double cutoff, double *p, double *f) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  if (n < 0 || (n > 0 && (!x || !q || !p || !f)) || ksize < 0 || !(alpha > 0) || !(sigma > 0) || !(cutoff >= 0))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_ewald: bad arguments");
+  const double *cy = c->p.cycle;
+  if (!(cy[0] > 0 && cy[1] > 0 && cy[2] > 0)) FMM_FAIL(FMM_ERR_ARG, "fmm_ewald: cycle must be > 0");
+  if (n == 0) return FMM_OK;
+  HIPCHK(hipSetDevice(c->p.device));
+  // bodies wrapped into the box (md_distributed.cxx:329), structure of arrays
+  std::vector<double> h(4 * (size_t)n);
+  for (size_t i = 0; i < (size_t)n; i++) {
+    for (int d = 0; d < 3; d++) {
+      double v = x[3 * i + d];
+      if (v < -cy[d] / 2) v += cy[d];
+      if (v > cy[d] / 2) v -= cy[d];
+      h[(size_t)d * n + i] = v;
+    }
+    h[3 * (size_t)n + i] = q[i];
+  }
+  // Ewald::initWaves (include/ewald.h:64-88): the half space of k, |k| <= ksize
+  std::vector<int> kv;
+  for (int l = 0; l <= ksize; l++)
+    for (int m = l == 0 ? 0 : -ksize; m <= ksize; m++)
+      for (int k = (l == 0 && m == 0) ? 1 : -ksize; k <= ksize; k++)
+        if (l * l + m * m + k * k <= ksize * ksize) kv.insert(kv.end(), {l, m, k});
+  const int nw = (int)(kv.size() / 3);
+  int rc;
+  if ((rc = upload(c, c->ew_b, h.data(), h.size()))) return rc;
+  if ((rc = ensure(c, c->ew_t, sizeof(double) * 4 * n))) return rc;
+  if ((rc = upload(c, c->ew_k, kv.data(), kv.size()))) return rc;
+  if ((rc = ensure(c, c->ew_s, sizeof(double2) * std::max(nw, 1)))) return rc;
+  HIPCHK(hipMemsetAsync(c->ew_t.p, 0, sizeof(double) * 4 * n, c->st));
+  const double *bx = c->ew_b.as<double>();
+  double *tx = c->ew_t.as<double>();
+  const fmmdev::DevBodies B{bx, bx + n, bx + 2 * (size_t)n, bx + 3 * (size_t)n,
+                            tx, tx + n, tx + 2 * (size_t)n, tx + 3 * (size_t)n};
+  HIPCHK(fmmdev::launch_ewald(B, n, c->ew_k.as<int>(), nw, c->ew_s.as<double2>(),
+                              make_double3(cy[0], cy[1], cy[2]), alpha, sigma, cutoff, c->st));
+  std::vector<double> o(4 * (size_t)n);
+  HIPCHK(hipMemcpyAsync(o.data(), c->ew_t.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (size_t i = 0; i < (size_t)n; i++) {  // TRG starts from the caller's p, f (md_distributed.cxx:331-334)
+    p[i] += o[i];
+    for (int d = 0; d < 3; d++) f[3 * i + d] += o[(size_t)(d + 1) * n + i];
   }
   return FMM_OK;
 }

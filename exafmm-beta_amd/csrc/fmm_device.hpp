@@ -109,6 +109,12 @@ hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, 
 // getDipole + dipoleCorrection on the bodies' targets; part: 3 * dipole_partials(n) doubles
 int dipole_partials(int n);
 hipError_t launch_dipole_correction(DevBodies B, int n, double *part, double coef, hipStream_t s);
+// Ewald sum (include/ewald.h) over the bodies B (positions inside the box):
+// wave part over the nw wave vectors kv (3 ints each; S: nw scratch values),
+// real part over minimum-image pairs within the cutoff, self term; adds into
+// B's targets
+hipError_t launch_ewald(DevBodies B, int n, const int *kv, int nw, double2 *S, double3 cyc, double alpha,
+                        double sigma, double cutoff, hipStream_t s);
 hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,
                          double *out, double eps2, int prange, double3 cyc, hipStream_t s);
 

@@ -28,7 +28,7 @@ SYMBOLS = [
     "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
     "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
-    "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts",
+    "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts", "fmm_ewald",
 ]
 
 FMM_FP64 = 0
@@ -133,6 +133,8 @@ def load():
         L.fmm_make_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         L.fmm_direct.argtypes = [vp, ctypes.c_int, _dp, _dp]
         L.fmm_md_evaluate.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.fmm_ewald.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, _dp, _dp]
         _up = ctypes.POINTER(ctypes.c_uint64)
         L.fmm_plan_list_counts.argtypes = [vp, _up, _ip, _ip]
         L.fmm_get_list_counts.argtypes = [vp, _up, _ip, _ip]
@@ -300,6 +302,20 @@ class FMM:
         self._check(load().fmm_md_evaluate(self._h, n, _ptr(x), _ptr(q), _ptr(p), _ptr(f)),
                     "fmm_md_evaluate")
         self.n = n
+        return p, f
+
+    def ewald(self, x, q, ksize, alpha, sigma, cutoff, p=None, f=None):
+        """FMM_Ewald (wrappers/md_distributed.cxx:310-360) on the device: returns
+        (p, f) with the Ewald potential and gradient added, period = cycle."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        n = len(q)
+        if len(x) != n:
+            raise ValueError("x and q lengths differ")
+        p = np.zeros(n) if p is None else np.array(p, dtype=np.float64)
+        f = np.zeros((n, 3)) if f is None else np.array(f, dtype=np.float64).reshape(n, 3)
+        self._check(load().fmm_ewald(self._h, n, _ptr(x), _ptr(q), ksize, alpha, sigma, cutoff, _ptr(p),
+                                     _ptr(f)), "fmm_ewald")
         return p, f
 
     def stats(self):

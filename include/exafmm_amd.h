@@ -206,6 +206,17 @@ int fmm_dipole_correction(fmm_ctx *c);
  * call (the reference keeps the bounds of its last FMM_Partition). */
 int fmm_md_evaluate(fmm_ctx *c, int n, const double *x, const double *q, double *p, double *f);
 
+/* FMM_Ewald (wrappers/md_distributed.cxx:310-360) on the device, one process:
+ * the Ewald sum of include/ewald.h for the n bodies x (wrapped into the box
+ * of the context's cycle; x itself is not changed) with charges q:
+ * Ewald::wavePart over the wave vectors of initWaves (|k| <= ksize), the real
+ * part (erfc-screened pairs within `cutoff`, each at its minimum image) and
+ * Ewald::selfTerm; p[i] += potential, f[3i..3i+2] += gradient, caller's
+ * order.  Uses the context's stream and its own device buffers; the bodies
+ * set for the FMM are not touched. */
+int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, double alpha, double sigma,
+              double cutoff, double *p, double *f);
+
 /* Traversal::direct (include/traversal.h:355-377): direct sum over all
  * bodies of the context (tree order) -- with images > 0 over the cube of
  * shifts (ix, iy, iz) * cycle, |i| <= sum_(l < images) neighbor^l -- at nt

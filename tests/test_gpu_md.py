@@ -107,3 +107,29 @@ def test_md_evaluate_empty_and_bad_arguments():
         assert p.shape == (0,) and fa.shape == (0, 3)
         with pytest.raises(ValueError):
             f.md_evaluate(np.zeros((3, 3)), np.zeros(2))
+
+
+def test_device_ewald_matches_oracle():
+    """fmm_ewald (FMM_Ewald on the device) equals the oracle's Ewald sum, with
+    bodies outside the box wrapped and the caller's p, f accumulated."""
+    x, q = _system(1500, 6, spread=0.6)
+    rng = np.random.default_rng(7)
+    p0, f0 = rng.random(1500), rng.random((1500, 3))
+    with _fmm().FMM(P=6, images=1, cycle=L) as f:
+        p, fa = f.ewald(x, q, 11, 10 / L, 0.25 / np.pi, L / 2, p0, f0)
+    xw, _ = O.wrap(x, L)
+    ref = O.ewald(xw, q, 11, 10 / L, 0.25 / np.pi, L / 2, L)
+    assert _rel(p - p0, ref[:, 0]) < 1e-12
+    assert _rel(fa - f0, ref[:, 1:]) < 1e-12
+
+
+def test_device_ewald_rock_salt():
+    """The rock-salt Madelung constant through the device Ewald sum."""
+    g = np.arange(4)
+    i, j, k = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([i, j, k], -1).reshape(-1, 3).astype(float) - 1.5
+    q = np.where((i + j + k).reshape(-1) % 2 == 0, 1.0, -1.0)
+    with _fmm().FMM(P=6, images=1, cycle=4.0) as f:
+        p, fa = f.ewald(x, q, 24, 3.0, 0.25 / np.pi, 2.0)
+    assert np.max(np.abs(q * p + 1.747564594633182)) < 1e-10
+    assert np.max(np.abs(fa)) < 1e-10
