@@ -898,7 +898,7 @@ int fmm_m2l(fmm_ctx *c) {
                                      c->M32.as<float2>(), c->st));
     if (c->use_tiles) {
       HIPCHK(fmmdev::launch_m2l_tile(c->p.P, c->mixed, c->ntile, c->t_tgt.as<int>(), c->t_ptr.as<int4>(),
-                                     c->t_slice.as<int4>(), c->t_src.as<int>(), c->t_lt.as<unsigned char>(),
+                                     c->t_slice.as<int4>(), c->t_src.as<int>(), c->t_lt.as<unsigned short>(),
                                      c->g_ops.p, c->cr.as<double>(), c->mixed ? c->M32.p : c->M.p,
                                      c->L.as<double2>(), c->st));
     } else if (c->mixed) {

@@ -59,7 +59,7 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
 int m2l_tile_targets(int P);
 bool m2l_tiles_preferred(int P, bool mixed);  // AUTO uses tiles (else chunks) at this order
 hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const int4 *items, const int4 *slices,
-                           const int *psrc, const unsigned char *plt, const void *ops, const double *cr,
+                           const int *psrc, const unsigned short *plt, const void *ops, const double *cr,
                            const void *M, double2 *L, hipStream_t s);
 // every group's dense operator as the tile kernel's MFMA A operands (from the
 // group harmonics); group_ops_bytes: bytes per group

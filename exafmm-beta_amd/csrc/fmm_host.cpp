@@ -995,7 +995,7 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
   for (int64_t q = 0; q < np; q++) pcount[tile_of[G.tgt[q]] + 1]++;
   for (int i = 0; i < nt; i++) pcount[i + 1] += pcount[i];
   bigvec<int> pgrp(np), psrc(np);
-  bigvec<uint8_t> plt(np);
+  bigvec<uint16_t> plt(np);
   std::vector<int> gorder(G.ngroup);
   for (int g = 0; g < G.ngroup; g++) gorder[g] = g;
   std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
@@ -1046,44 +1046,67 @@ M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int 
         const int64_t kk = fill[tile_of[c]]++;
         pgrp[kk] = g;
         psrc[kk] = G.src[q];
-        plt[kk] = (uint8_t)local[c];
+        plt[kk] = (uint16_t)(local[c] | (G.refl[g] << 8));
       }
     }
   });
   // tiles in key order (neighbours adjacent); the work items built from them
-  // are dealt to the XCDs by split_m2l_tiles.  Slices: runs of at most SW
-  // pairs of one group inside a tile, pairs kept in place.
+  // are dealt to the XCDs by split_m2l_tiles.  Slices: the pairs of one
+  // representative inside a tile (every reflection of it) are dealt into
+  // S = max(ceil(m / SW), most pairs of one target) slices of distinct
+  // targets -- the pairs sorted by target, pair i to slice i mod S -- so a
+  // slice mixes reflections but never repeats a target.
   T.ntile = nt;
   T.tgt.assign((size_t)T.ntile * TT, -1);
   for (size_t i = 0; i < ord.size(); i++) T.tgt[i] = tg[ord[i]];
-  auto slices_of = [&](int b, int *out) {  // count (out == nullptr) or write the slices of tile b
-    int ns = 0;
-    for (int64_t k = pcount[b]; k < pcount[b + 1];) {
-      const int g = pgrp[k];
-      int64_t e = k;
-      while (e < pcount[b + 1] && pgrp[e] == g && e - k < SW) e++;
-      if (out) {
-        int *o = out + 4 * (size_t)ns;
-        o[0] = G.canon[g];
-        o[1] = (int)k;
-        o[2] = (int)(e - k);
-        o[3] = (int)G.refl[g];
-      }
-      ns++;
-      k = e;
-    }
-    return ns;
-  };
-  T.ptr.assign(nt + 1, 0);
+  std::vector<std::vector<int>> tsl(nt);  // per tile: (representative, first, count) per slice
   const int RT = std::min(R, std::max(nt, 1));
   par(RT, [&](int r) {
-    for (int b = (int)((int64_t)nt * r / RT); b < (int)((int64_t)nt * (r + 1) / RT); b++) T.ptr[b + 1] = slices_of(b, nullptr);
+    std::vector<int> cnt(TT + 1), seq;
+    std::vector<int> bsrc;
+    std::vector<uint16_t> blt;
+    for (int b = (int)((int64_t)nt * r / RT); b < (int)((int64_t)nt * (r + 1) / RT); b++) {
+      std::vector<int> &out = tsl[b];
+      for (int64_t k = pcount[b]; k < pcount[b + 1];) {
+        const int rep = G.canon[pgrp[k]];
+        int64_t e = k;
+        while (e < pcount[b + 1] && G.canon[pgrp[e]] == rep) e++;
+        const int m = (int)(e - k);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        int most = 0;
+        for (int64_t q = k; q < e; q++) most = std::max(most, ++cnt[(plt[q] & 255) + 1]);
+        const int S = std::max((m + SW - 1) / SW, most);
+        for (int u = 0; u < TT; u++) cnt[u + 1] += cnt[u];
+        seq.resize(m);
+        for (int64_t q = k; q < e; q++) seq[cnt[plt[q] & 255]++] = (int)(q - k);
+        bsrc.assign(psrc.begin() + k, psrc.begin() + e);
+        blt.assign(plt.begin() + k, plt.begin() + e);
+        int64_t o = k;
+        for (int j = 0; j < S; j++) {
+          const int64_t first = o;
+          for (int i = j; i < m; i += S) {
+            psrc[o] = bsrc[seq[i]];
+            plt[o] = blt[seq[i]];
+            o++;
+          }
+          out.insert(out.end(), {rep, (int)first, (int)(o - first)});
+        }
+        k = e;
+      }
+    }
   });
-  for (int b = 0; b < nt; b++) T.ptr[b + 1] += T.ptr[b];
+  T.ptr.assign(nt + 1, 0);
+  for (int b = 0; b < nt; b++) T.ptr[b + 1] = T.ptr[b] + (int)(tsl[b].size() / 3);
   T.slice.resize(4 * (size_t)T.ptr[nt]);
   par(RT, [&](int r) {
     for (int b = (int)((int64_t)nt * r / RT); b < (int)((int64_t)nt * (r + 1) / RT); b++)
-      slices_of(b, T.slice.data() + 4 * (size_t)T.ptr[b]);
+      for (size_t i = 0; i < tsl[b].size() / 3; i++) {
+        int *o = T.slice.data() + 4 * ((size_t)T.ptr[b] + i);
+        o[0] = tsl[b][3 * i];
+        o[1] = tsl[b][3 * i + 1];
+        o[2] = tsl[b][3 * i + 2];
+        o[3] = 0;
+      }
   });
   T.src = std::move(psrc);
   T.lt = std::move(plt);

@@ -138,11 +138,11 @@ void list_counts(const Tree &t, const Lists &L, uint64_t *icell, int *nump2p, in
 // Target-stationary layout of the grouped M2L pairs: targets are sorted by
 // (level, octant within the parent, cell) -- cells of one octant class share
 // their offsets -- and cut into tiles of TT targets.  A tile's pairs are
-// ordered by (representative, reflection) and cut into slices of at most SW
-// pairs of one group, so a
-// block can apply each group's operator to whole slices and keep the tile's
-// local expansions in LDS until one final store.  Tiles are listed in launch
-// order (dealt to the 8 XCDs in contiguous runs; empty tiles pad).
+// ordered by representative and cut into slices of at most SW pairs with
+// distinct targets (a slice may mix the representative's reflections), so a
+// block can apply each representative's operator to whole slices and keep the
+// tile's local expansions in LDS until one final store.  Tiles are listed in
+// launch order (dealt to the 8 XCDs in contiguous runs; empty tiles pad).
 // A block runs one work item: at most `seg` consecutive slices of one tile.
 // A tile split over several items adds its results (atomics); a tile in one
 // item stores them.
@@ -150,10 +150,10 @@ struct M2LTiles {
   int TT = 0, ntile = 0;
   std::vector<int> tgt;        // ntile*TT target cells (-1 pads), launch order
   std::vector<int> ptr;        // ntile+1 offsets into the slices
-  std::vector<int> slice;      // 4 per slice: representative, first pair, count, reflection bits
-                               // (the slice's pairs belong to one group)
+  std::vector<int> slice;      // 4 per slice: representative, first pair, count, 0
   bigvec<int> src;             // source cell of every pair
-  bigvec<uint8_t> lt;          // target of every pair as its index in the tile
+  bigvec<uint16_t> lt;         // every pair: target index in the tile (bits 0-7) and the
+                               // reflection bits of its group (bits 8-10)
   std::vector<int> item;       // 4 per work item: tile, first slice, end slice, split (0/1)
 };
 M2LTiles build_m2l_tiles(const Tree &t, const M2LGroups &G, int TT, int SW, int seg = 64);

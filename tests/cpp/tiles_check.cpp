@@ -1,7 +1,8 @@
 // Host-side invariants of the target-stationary M2L layout (fmm::build_m2l_tiles
 // and its merge): every grouped pair appears exactly once, in the tile that
-// owns its target; a slice holds at most 16 pairs of one group with distinct
-// targets and names that group's representative and reflection; every
+// owns its target; a slice holds at most 16 pairs with distinct targets whose
+// groups all have the slice's representative, each pair carrying its group's
+// reflection bits; every
 // group's offset is its representative's reflected by its bits; a tile's
 // targets are distinct and every target has one tile.  Prints "ok" or the
 // first violation.
@@ -42,19 +43,18 @@ static int check(const Tree &t, const M2LGroups &G, const M2LTiles &T) {
       if (!tiles_t.insert(c).second || !seen_t.insert(c).second) return printf("target %d twice\n", c), 1;
     }
     for (int s = T.ptr[b]; s < T.ptr[b + 1]; s++) {
-      const int rep = T.slice[4 * s], p0 = T.slice[4 * s + 1], n = T.slice[4 * s + 2], rf = T.slice[4 * s + 3];
+      const int rep = T.slice[4 * s], p0 = T.slice[4 * s + 1], n = T.slice[4 * s + 2];
       if (n < 1 || n > 16) return printf("slice size %d\n", n), 1;
-      int g = -1;  // the slice's group, from its first pair
+      if (T.slice[4 * s + 3] != 0) return printf("slice field 3 not zero\n"), 1;
       std::set<int> lts;
       for (int k = p0; k < p0 + n; k++) {
-        const int lt = T.lt[k];
+        const int lt = T.lt[k] & 255, rf = T.lt[k] >> 8;
         if (!lts.insert(lt).second) return printf("target repeated in a slice\n"), 1;
         const int c = T.tgt[(size_t)b * T.TT + lt];
         auto it = want.find({c, T.src[k]});
         if (c < 0 || it == want.end()) return printf("pair not grouped\n"), 1;
-        if (g < 0) g = it->second;
-        if (it->second != g) return printf("slice mixes groups\n"), 1;
-        if (G.canon[g] != rep || G.refl[g] != rf) return printf("slice names the wrong representative\n"), 1;
+        const int g = it->second;
+        if (G.canon[g] != rep || G.refl[g] != rf) return printf("pair names the wrong representative\n"), 1;
         want.erase(it);
         npairs++;
       }
