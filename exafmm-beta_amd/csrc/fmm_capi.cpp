@@ -642,6 +642,10 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     if (c->use_tiles) {
       if (tiles.TT == 0) tiles = fmm::build_m2l_tiles(t, G, fmmdev::m2l_tile_targets(c->p.P), 16);
       c->ntile = (int)(tiles.item.size() / 4);  // work items
+      // the tile kernel holds an item's slice descriptors one per lane
+      for (size_t i = 0; i < tiles.item.size(); i += 4)
+        if (tiles.item[i + 2] - tiles.item[i + 1] > fmmdev::kM2LItemSlices)
+          FMM_FAIL(FMM_ERR_UNSUPPORTED, "M2L tiles: a work item holds more than 64 slices");
       UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item) UP(t_slice, tiles.slice) UP(t_src, tiles.src)
       UP(t_lt, tiles.lt) UP(c_dx, G.cdx)
       if (c->mixed) UP(c_sc, G.cscale)

@@ -43,6 +43,7 @@ hipError_t launch_m2l(int P, DevCells C, const int *targets, int ntarget, const 
 // value).  Available for the orders where m2l_mfma_supported(P).
 bool m2l_mfma_supported(int P);
 constexpr int kM2LChunkPairs = 512;  // pairs per M2L chunk (one block)
+constexpr int kM2LItemSlices = 64;   // most slices in one M2L tile work item (one per lane)
 hipError_t launch_m2l_mfma(int P, const int4 *chunks, int nchunk, const double *gdx, const int *ptgt,
                            const int *psrc, const double2 *M, double2 *L, hipStream_t s);
 // L2L into every cell in `cells` from its parent (laplace.h:324-357); accumulates
