@@ -787,35 +787,39 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     for (int d = 0; d < 3; d++) G.dx[3 * (size_t)g + d] = (double)kvec[3 * (size_t)g + d] * u - sh[d];
   }
   G.ngroup = ng;
-  // reflection classes (non-periodic offsets only)
+  // reflection classes: a group is reflected in every axis where its
+  // translation vector (periodic: including the image shift) is negative;
+  // reflecting an axis negates the lattice offset and the image's shift
+  // along it, so the reflected vector is the exact negation and groups whose
+  // vectors differ only in signs share one representative
   G.canon.resize(ng);
   G.refl.assign(ng, 0);
   {
     KeyIndex2 index;
     for (int g = 0; g < ng; g++) {
       const int64_t *k = &kvec[3 * (size_t)g];
+      const double *t = &G.dx[3 * (size_t)g];
       uint8_t r = 0;
       int64_t a[3];
+      int ic[3] = {gimg[g] / 9 - 1, gimg[g] / 3 % 3 - 1, gimg[g] % 3 - 1};
       for (int d = 0; d < 3; d++) {
         a[d] = k[d];
-        if (!cycle && k[d] < 0) {
+        if (cycle ? t[d] < 0.0 : k[d] < 0) {
           a[d] = -k[d];
+          ic[d] = -ic[d];
           r |= (uint8_t)(1 << d);
         }
       }
-      const uint64_t key = cycle ? (uint64_t)g  // periodic: every group is its own class
-                                 : ((uint64_t)gexp[g] << 57) | ((uint64_t)(a[0] + lim) << 38) |
-                                       ((uint64_t)(a[1] + lim) << 19) | (uint64_t)(a[2] + lim);
+      const int imgc = cycle ? (ic[0] + 1) * 9 + (ic[1] + 1) * 3 + (ic[2] + 1) : 0;  // no images: 0
+      const uint64_t key = ((uint64_t)gexp[g] << 57) | ((uint64_t)(a[0] + lim) << 38) |
+                           ((uint64_t)(a[1] + lim) << 19) | (uint64_t)(a[2] + lim);
+      const uint64_t key2 = ((uint64_t)imgc << 20) | (gkey2[g] & ((1ull << 20) - 1));
       bool isnew;
-      const int cg = index.get(key, cycle ? 0 : gkey2[g], isnew);
+      const int cg = index.get(key, key2, isnew);
       G.canon[g] = cg;
       G.refl[g] = r;
       if (isnew) {
-        const double u = R0 / std::ldexp(1.0, gexp[g]);
-        if (cycle)
-          G.cdx.insert(G.cdx.end(), {G.dx[3 * (size_t)g], G.dx[3 * (size_t)g + 1], G.dx[3 * (size_t)g + 2]});
-        else
-          G.cdx.insert(G.cdx.end(), {(double)a[0] * u, (double)a[1] * u, (double)a[2] * u});
+        G.cdx.insert(G.cdx.end(), {(r & 1) ? -t[0] : t[0], (r & 2) ? -t[1] : t[1], (r & 4) ? -t[2] : t[2]});
         G.cscale.insert(G.cscale.end(), {sc[2 * (size_t)g], sc[2 * (size_t)g + 1]});
       }
     }

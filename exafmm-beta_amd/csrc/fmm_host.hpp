@@ -116,8 +116,9 @@ struct M2LGroups {
   // y: conj, z: (-1)^(n+m), on both the multipole columns and the local
   // rows), so every group maps to a representative with non-negative offset
   // components (canon) and the reflections that lead there (refl: bit d set
-  // when component d was negative).  Periodic groups (offsets carrying an
-  // image shift) are their own representatives.
+  // when component d was negative).  Periodic groups reflect their image
+  // shift with the offset, so the sign test is on the whole translation
+  // vector (offset minus image shift).
   int ncanon = 0;
   std::vector<int> canon;          // per group
   std::vector<uint8_t> refl;       // per group
