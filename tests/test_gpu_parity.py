@@ -215,7 +215,7 @@ def test_input_order_and_determinism():
     h.close()
 
 
-@pytest.mark.parametrize("P", [6, 8, 10, 12])
+@pytest.mark.parametrize("P", [6, 7, 8, 9, 10, 11, 12])
 def test_grouped_m2l_matches_per_target(P):
     x, q = _inputs(20000, "plummer", seed=4)
     F = _fmm()
