@@ -80,6 +80,7 @@ struct fmm_ctx {
   // offset-grouped M2L: chunks (GROUPED) or target tiles (AUTO)
   bool use_mfma = false, use_tiles = false;
   int nchunk = 0, ntile = 0;
+  int p2p_nwide = 0;  // P2P targets [0, p2p_nwide) are leaves of >= 16 bodies
   fmm::M2LGroups groups;
   DBuf g_tgt, g_src, g_dx, g_chunk, r_ptr, r_src, r_tgt;
   DBuf t_tgt, t_ptr, t_slice, t_src, t_lt, g_y, g_ops, c_dx, c_sc;
@@ -559,7 +560,13 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   UP(iparent, t.iparent)
   UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
-  UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, L.p2p_targets)
+  // P2P targets, leaves of >= 16 bodies first (the kernel's targets per lane
+  // follow the leaf size, so every leaf is summed the same way in any batch)
+  std::vector<int> p2p_order(L.p2p_targets);
+  c->p2p_nwide = (int)(std::stable_partition(p2p_order.begin(), p2p_order.end(),
+                                             [&](int cell) { return t.nbody[cell] >= fmmdev::kP2PWideLeaf; }) -
+                       p2p_order.begin());
+  UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, p2p_order)
   // M2M / L2L operators: every level's radius uniform and every child at
   // X_parent + R_child (+-1, +-1, +-1) (up to rounding of the centres)
   c->xl = fmmdev::xlate_mfma_supported(c->p.P) && nc > 1;
@@ -877,7 +884,7 @@ int p2p_on(fmm_ctx *c, hipStream_t s) {
   if (c->timing) HIPCHK(hipEventRecord(c->kev[2], s));
   HIPCHK((c->mixed ? fmmdev::launch_p2p32 : fmmdev::launch_p2p)(
       dev_bodies(c), dev_cells(c), c->p2p_tgt.as<int>(), (int)c->lists.p2p_targets.size(),
-      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->tree.adopted ? 1 : 0, s));
+      c->p2p_ptr.as<int>(), c->p2p_src.as<int>(), xper, c->p.eps2, c->tree.adopted ? 1 : 0, c->p2p_nwide, s));
   if (c->timing) HIPCHK(hipEventRecord(c->kev[3], s));
   return FMM_OK;
 }

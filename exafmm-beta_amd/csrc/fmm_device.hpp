@@ -100,10 +100,13 @@ bool m2l_mixed_supported(int P);
 hipError_t launch_m_to_f32(int P, const double2 *M, const double *R, int ncell, float2 *M32, hipStream_t s);
 hipError_t launch_m2l_mfma32(int P, const int4 *chunks, int nchunk, const double *gdx, const double2 *gscale,
                              const int *ptgt, const int *psrc, const float2 *M32, double2 *L, hipStream_t s);
+// targets[0, nwide) are leaves of at least kP2PWideLeaf bodies (run with four
+// targets per lane), the rest run with one
+constexpr int kP2PWideLeaf = 16;
 hipError_t launch_p2p32(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
-                        const int *src, double3 xper, double eps2, int mask_all, hipStream_t s);
+                        const int *src, double3 xper, double eps2, int mask_all, int nwide, hipStream_t s);
 hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, const int *ptr,
-                      const int *src, double3 xper, double eps2, int mask_all, hipStream_t s);
+                      const int *src, double3 xper, double eps2, int mask_all, int nwide, hipStream_t s);
 
 // Direct sum (include/traversal.h:355-377) of all n sources at nt targets
 // (SoA txyz); writes tpot/tax/tay/taz.
