@@ -8,11 +8,13 @@ and acceleration vs a direct sum on sampled targets, the roofline of the
 dominant kernel and a CPU baseline (exafmm-beta's own operators, one core, on
 a bounded sample of the same lists).
 
-Default workload: N = 2^20 Plummer bodies (unit scale radius, r < 10,
-q = 1/N), P = 10, theta = 0.4, ncrit = 64, FP64.  The *-mixed workloads
-(N = 2^23 uniform cube, P = 8 / 12) run the mixed-precision solve and the FP64
-solve of the same system, and report both with their errors against the FP64
-direct sum.
+Default workload: N = 2^23 bodies uniform in [-1,1]^3, zero-mean charges in
+[-1,1], P = 12, theta = 0.4, ncrit = 64.  `value` is the FP64 solve (the
+reference's precision); the mixed-precision solve of the same system (same
+tree and lists) is reported beside it (`mixed_same_system`,
+`mixed_over_fp64`, their rel-L2 distance), both with their errors against
+the FP64 direct sum, and the same system at theta = 0.35
+(`accuracy_point`), where both reach rel-L2 <= 1e-6.
 
   python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
   python bench.py --copy-probe MIB   # one device copy of MIB MiB (HBM counter calibration)
@@ -36,7 +38,17 @@ PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X FP32 vector (the packed-FP32 rate)
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
-    # N = 2^20 Plummer, FP64, P = 10: the single-GPU FP64 solve
+    # N = 2^23 uniform cube, P = 12: the FP64 solve (`value`) and the mixed
+    # solve of the same system (same tree and lists) beside it, plus the same
+    # system at theta = 0.35, where both reach rel-L2 <= 1e-6 against the
+    # direct sum
+    "cube8m-p12": dict(n=8388608, dist="cube1", P=12, theta=0.4, ncrit=64, nspawn=5000,
+                       ntarget=1000, precision="fp64", compare_mixed=True, accurate_theta=0.35,
+                       desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=12, theta=0.4, ncrit=64"),
+    "cube8m-p8": dict(n=8388608, dist="cube1", P=8, theta=0.4, ncrit=64, nspawn=5000,
+                      ntarget=1000, precision="fp64", compare_mixed=True,
+                      desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=8, theta=0.4, ncrit=64"),
+    # N = 2^20 Plummer, FP64, P = 10
     "plummer1m-p10-fp64": dict(n=1 << 20, dist="plummer1", P=10, theta=0.4, ncrit=64, nspawn=5000,
                                ntarget=1000, precision="fp64",
                                desc="N=2^20 Plummer (a=1, r<10), q=1/N, P=10, theta=0.4, ncrit=64"),
@@ -44,17 +56,8 @@ WORKLOADS = {
     "cube32k-p10-fp64": dict(n=32768, dist="cube1", P=10, theta=0.4, ncrit=64, nspawn=5000,
                              ntarget=100, precision="fp64",
                              desc="N=32768 cube [-1,1]^3, zero-mean q in [-1,1], P=10, theta=0.4"),
-    # N = 2^23 uniform cube, mixed precision, with the FP64 solve of the same
-    # system measured beside it (same tree and lists)
-    "cube8m-p8-mixed": dict(n=8388608, dist="cube1", P=8, theta=0.4, ncrit=64, nspawn=5000,
-                            ntarget=1000, precision="mixed",
-                            desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=8, theta=0.4, "
-                                 "ncrit=64; FP32 P2P/M2L, FP64 P2M/M2M/L2L/L2P"),
-    "cube8m-p12-mixed": dict(n=8388608, dist="cube1", P=12, theta=0.4, ncrit=64, nspawn=5000,
-                             ntarget=1000, precision="mixed",
-                             desc="N=8388608 cube [-1,1]^3, zero-mean q in [-1,1], P=12, theta=0.4, "
-                                  "ncrit=64; FP32 P2P/M2L, FP64 P2M/M2M/L2L/L2P"),
 }
+DEFAULT_WORKLOAD = "cube8m-p12"
 
 # Serving stream: independent systems, N = floor(2^U[12,18]), uniform cube or
 # Plummer with equal probability, zero-mean uniform charges, P = 8, theta =
@@ -197,6 +200,13 @@ def m2l_flops(P, pairs):
     return 8.0 * (P * (P + 1) // 2) * P * P * pairs
 
 
+def m2l_exec_flops(P, pairs):
+    """M2L work as the kernels execute it: one dense real P^2 x P^2 operator
+    applied per pair (2 P^4 flop), without the padding of partly filled
+    slices or chunks."""
+    return 2.0 * P ** 4 * pairs
+
+
 def p2p_flops(body_pairs):
     """Algorithmic P2P work: 20 flops per body pair (potential + gradient)."""
     return 20.0 * body_pairs
@@ -211,65 +221,80 @@ def rel_l2(ref, got):
     return math.sqrt(pd / pn), math.sqrt(ad / an)
 
 
-def cpu_baseline(x, q, w, budget_s):
-    """exafmm-beta's own Kernel operators (oracle/_ref, built from the reference
-    sources) timed on one host core over a stride sample of the same M2L and
-    P2P lists, plus the full upward and downward passes; scaled to the full
-    solve.  Falls back to the oracle's C restatement when _ref is absent."""
+def cpu_baseline(x, q, w, plan, budget_s):
+    """exafmm-beta's own Kernel operators (oracle/_ref, built from the
+    reference sources) timed on one host core on a stride sample of every
+    pass of the same solve -- P2M and L2P of every s-th leaf, M2M of every
+    s-th internal cell, L2L of every s-th cell, and every k-th pair of the
+    M2L and P2P lists -- each phase scaled by its share of the whole work.
+    The tree and lists are the host plan's (identical to the oracle's).
+    Falls back to the oracle's C restatement (whole upward/downward passes)
+    when _ref is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    t = O.Tree(x, w["ncrit"])
-    xs, qs = x[t.perm], q[t.perm]
-    ci, cj, ty = t.traverse(w["theta"], w["nspawn"])
-    m2l = np.nonzero(ty == 0)[0]
-    p2p = np.nonzero(ty == 1)[0]
-    body_pairs = t.nbody[ci[p2p]].astype(np.int64) * t.nbody[cj[p2p]]
+    tr = plan.tree()
+    xs, qs = x[tr["perm"]], q[tr["perm"]]
+    mp, ms, pp, ps = plan.lists()
+    nc = len(tr["nbody"])
+    mi = np.repeat(np.arange(nc, dtype=np.int32), np.diff(mp))
+    pi = np.repeat(np.arange(nc, dtype=np.int32), np.diff(pp))
+    body_pairs = tr["nbody"][pi].astype(np.int64) * tr["nbody"][ps]
     P = w["P"]
-    # per-pair cost estimates (one core): M2L ~17 us at P=10 scaling as P^4,
-    # P2P ~3.4 ns per body pair; pick strides so the sample fits the budget
-    est_m2l = 17e-6 * (P / 10.0) ** 4 * len(m2l)
+    n = len(x)
+    # per-unit cost estimates on one core (M2L ~17 us at P=10 scaling as P^4,
+    # P2P ~3.4 ns per body pair, a pass ~2 us per body at P=10 scaling as P^2)
+    est_m2l = 17e-6 * (P / 10.0) ** 4 * len(mi)
     est_p2p = 3.4e-9 * float(body_pairs.sum())
-    s_m2l = max(1, int(math.ceil(est_m2l / (0.6 * budget_s))))
+    est_pass = 2 * 2e-6 * (P / 10.0) ** 2 * n + 2 * 20e-6 * (P / 10.0) ** 4 * nc
+    s_m2l = max(1, int(math.ceil(est_m2l / (0.5 * budget_s))))
     s_p2p = max(1, int(math.ceil(est_p2p / (0.25 * budget_s))))
-    sm, sp = m2l[::s_m2l], p2p[::s_p2p]
-    frac_m2l = len(m2l) / max(len(sm), 1)
-    frac_p2p = float(body_pairs.sum()) / max(float(body_pairs[::s_p2p].sum()), 1.0)
+    s_pass = max(1, int(math.ceil(est_pass / (0.25 * budget_s))))
+    sm, sp = np.arange(0, len(mi), s_m2l), np.arange(0, len(pi), s_p2p)
     if O.ref_available():
-        k = O.Kernel(P, use_ref=True)
-        tu, tm, tp, td = O.ref_time(t, k, xs, qs, (ci[sm], cj[sm]), (ci[sp], cj[sp]))
-        kind = "reference"
+        t, work = O.ref_time_sampled(O.Kernel(P, use_ref=True), tr, xs, qs, s_pass, (mi[sm], ms[sm]),
+                                     (pi[sp], ps[sp]))
+        full = [float(n), float(tr["nchild"].sum()), float(len(mi)), float(body_pairs.sum()),
+                float(nc - 1), float(n)]
+        phase = {k: t[i] * full[i] / max(work[i], 1.0)
+                 for i, k in enumerate(("p2m", "m2m", "m2l", "p2p", "l2l", "l2p"))}
+        total = sum(phase.values())
+        kind, sampled = "reference", float(t.sum())
     else:
+        tree = O.Tree(x, w["ncrit"])
         k = O.Kernel(P)
         t0 = time.perf_counter()
-        M = t.upward(k, xs, qs)
+        M = tree.upward(k, xs, qs)
         t1 = time.perf_counter()
-        trg, Lc = t.eval_lists(k, xs, qs, ci[sm], cj[sm], ty[sm], M)
+        trg, Lc = tree.eval_lists(k, xs, qs, mi[sm], ms[sm], np.zeros(len(sm), np.int32), M)
         t2 = time.perf_counter()
-        t.eval_lists(k, xs, qs, ci[sp], cj[sp], ty[sp], M, trg=trg)
+        tree.eval_lists(k, xs, qs, pi[sp], ps[sp], np.ones(len(sp), np.int32), M, trg=trg)
         t3 = time.perf_counter()
-        t.downward(k, xs, Lc, trg)
+        tree.downward(k, xs, Lc, trg)
         t4 = time.perf_counter()
-        tu, tm, tp, td = t1 - t0, t2 - t1, t3 - t2, t4 - t3
-        kind = "port"
-    total = tu + td + tm * frac_m2l + tp * frac_p2p
+        phase = {"upward": t1 - t0, "m2l": (t2 - t1) * len(mi) / max(len(sm), 1),
+                 "p2p": (t3 - t2) * float(body_pairs.sum()) / max(float(body_pairs[sp].sum()), 1.0),
+                 "downward": t4 - t3}
+        total = sum(phase.values())
+        kind, sampled = "port", t4 - t0
     return {
-        "value": len(x) / total,
+        "value": n / total,
         "unit": "particles/s",
         "cores": 1,
         "kind": kind,
-        "sample": ("%s: full upward+downward passes, every %d-th of %d M2L pairs and every %d-th "
-                   "of %d P2P pairs timed on one core (%.1f s) and scaled to the full solve "
-                   "(est. %.1f s)" % (w["desc"], s_m2l, len(m2l), s_p2p, len(p2p),
-                                      tu + tm + tp + td, total)),
-        "phase_s": {"upward": tu, "m2l_sample": tm, "p2p_sample": tp, "downward": td},
+        "sample": ("%s: every %d-th cell of each pass (P2M, M2M, L2L, L2P), every %d-th of %d M2L "
+                   "pairs and every %d-th of %d P2P pairs timed on one core (%.1f s) and each phase "
+                   "scaled to the whole solve (est. %.1f s per solve at N=%d)"
+                   % (w["desc"], s_pass, s_m2l, len(mi), s_p2p, len(pi), sampled, total, n)),
+        "n": n,
+        "phase_s": phase,
     }
 
 
-def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0):
+def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0, theta=None):
     """Upload one system (host plan shared), run `warmup` + `steps` solves, and
     return timings, kernel figures and the accuracy on sampled targets."""
     n, P = w["n"], w["P"]
-    f = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], m2l_mode=m2l_mode,
+    f = F.FMM(P=P, theta=theta or w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], m2l_mode=m2l_mode,
               precision=F.FMM_MIXED if precision == "mixed" else F.FMM_FP64)
     f.set_bodies_plan(plan, q)
     for _ in range(warmup):
@@ -301,6 +326,7 @@ def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0):
     f.close()
     m2l_ms, p2p_ms = float(np.mean(km)), float(np.mean(kp))
     fm = m2l_flops(P, st["m2l_pairs"])
+    fe = m2l_exec_flops(P, st["m2l_pairs"])
     fp = p2p_flops(st["p2p_body_pairs"])
     peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
     return {
@@ -309,7 +335,9 @@ def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0):
                      "reference": "FP64 direct sum (Traversal::direct) on the GPU, IEEE sqrt/div"},
         "kernels": {
             "m2l": {"avg_ms": m2l_ms, "alg_flops": fm, "tflops": fm / m2l_ms * 1e-9,
-                    "peak_tflops": peak, "frac": fm / m2l_ms * 1e-9 / peak},
+                    "peak_tflops": peak, "frac": fm / m2l_ms * 1e-9 / peak,
+                    "exec_flops": fe, "exec_tflops": fe / m2l_ms * 1e-9,
+                    "frac_executed": fe / m2l_ms * 1e-9 / peak},
             "p2p": {"avg_ms": p2p_ms, "alg_flops": fp, "tflops": fp / p2p_ms * 1e-9,
                     "peak_tflops": PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_VALU_TFLOPS,
                     "frac": fp / p2p_ms * 1e-9 / (PEAK_FP64_TFLOPS if precision == "fp64"
@@ -394,13 +422,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plummer1m-p10-fp64",
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
                     choices=sorted(WORKLOADS) + sorted(SERVE) + sorted(MD))
     ap.add_argument("--requests", type=int, default=0,
                     help="serve-*: number of requests (default 2048); md-*: number of steps (default 1000)")
     ap.add_argument("--m2l-mode", type=int, default=0, choices=(0, 1, 2, 3),
                     help="M2L execution: 0 auto, 1 per target, 2 grouped chunks, 3 target tiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy-point", action="store_true",
+                    help="skip the smaller-theta solve that reaches rel-L2 <= 1e-6")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter calibration)")
@@ -449,10 +479,16 @@ def main():
     kernels = r["kernels"]
     dom = "m2l" if kernels["m2l"]["avg_ms"] >= kernels["p2p"]["avg_ms"] else "p2p"
     kd = kernels[dom]
-    # M2L runs on MFMA (grouped path: FP64 MFMA, or FP32 MFMA when mixed)
+    # M2L runs on MFMA (FP64 MFMA, or FP32 MFMA when mixed); achieved/frac use
+    # the algorithmic count, *_executed the dense real form the MFMAs execute
     roofline = {"bound": "mfma", "kernel": dom, "achieved": kd["tflops"], "peak": kd["peak_tflops"],
                 "unit": "TFLOP/s", "frac": kd["frac"],
                 "traffic": traffic_from_profile(args.workload, dom)}
+    if dom == "m2l":
+        roofline.update(achieved_executed=kd["exec_tflops"], frac_executed=kd["frac_executed"],
+                        count="achieved: 8*NTERM*P^2 flop per M2L pair (the reference's complex "
+                              "NTERM x P^2 product); achieved_executed: 2*P^4 per pair (the dense "
+                              "real operator applied on MFMA, padding not counted)")
     host_ms = st["host_bounds_ms"] + st["host_tree_ms"] + st["host_lists_ms"] + st["host_upload_ms"]
     value = n * args.steps / r["wall_s"]
     out = {
@@ -487,20 +523,41 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    if w["precision"] == "mixed":
-        # the FP64 solve of the same system, for "mixed vs FP64 at equal L2 error"
-        steps64 = max(1, args.steps // 2)
-        r64 = solve_and_time(F, w, plan, x, q, "fp64", steps64, 1, log, args.m2l_mode)
-        v64 = n * steps64 / r64["wall_s"]
-        out["fp64_same_system"] = {
-            "value": v64, "ms_per_step": 1e3 * r64["wall_s"] / steps64,
-            "latency_ms": {"p50": 1e3 * float(np.percentile(r64["lat"], 50)),
-                           "p99": 1e3 * float(np.percentile(r64["lat"], 99))},
-            "accuracy": r64["accuracy"], "kernels": r64["kernels"]}
-        out["mixed_over_fp64"] = value / v64
-        out["mixed_vs_fp64_rel_l2"] = dict(zip(("pot", "acc"), rel_l2(r64["trg"], r["trg"])))
+    if w.get("compare_mixed"):
+        # the mixed-precision solve of the same system (same tree and lists),
+        # for "mixed vs FP64 at equal L2 error"
+        rm = solve_and_time(F, w, plan, x, q, "mixed", args.steps, args.warmup, log, args.m2l_mode)
+        vm = n * args.steps / rm["wall_s"]
+        out["mixed_same_system"] = {
+            "value": vm, "ms_per_step": 1e3 * rm["wall_s"] / args.steps,
+            "latency_ms": {"p50": 1e3 * float(np.percentile(rm["lat"], 50)),
+                           "p99": 1e3 * float(np.percentile(rm["lat"], 99))},
+            "device_ms": float(np.mean(rm["dev"])), "accuracy": rm["accuracy"], "kernels": rm["kernels"],
+            "roofline": {"bound": "mfma", "kernel": "m2l", "achieved": rm["kernels"]["m2l"]["tflops"],
+                         "peak": rm["kernels"]["m2l"]["peak_tflops"], "unit": "TFLOP/s",
+                         "frac": rm["kernels"]["m2l"]["frac"],
+                         "achieved_executed": rm["kernels"]["m2l"]["exec_tflops"],
+                         "frac_executed": rm["kernels"]["m2l"]["frac_executed"]}}
+        out["mixed_over_fp64"] = vm / value
+        out["mixed_vs_fp64_rel_l2"] = dict(zip(("pot", "acc"), rel_l2(r["trg"], rm["trg"])))
+    at = w.get("accurate_theta")
+    if at and not args.no_accuracy_point:
+        # the same system at a smaller theta, where both precisions reach
+        # rel-L2 <= 1e-6 against the direct sum
+        plan2 = F.Plan(x, ncrit=w["ncrit"], theta=at, nspawn=w["nspawn"])
+        k = max(2, args.steps // 3)
+        pts = {}
+        for prec in ("fp64", "mixed"):
+            ra = solve_and_time(F, w, plan2, x, q, prec, k, 1, log, args.m2l_mode, theta=at)
+            pts[prec] = {"value": n * k / ra["wall_s"], "ms_per_step": 1e3 * ra["wall_s"] / k,
+                         "accuracy": ra["accuracy"], "m2l_pairs": ra["st"]["m2l_pairs"],
+                         "p2p_body_pairs": ra["st"]["p2p_body_pairs"]}
+        pts["theta"] = at
+        pts["mixed_over_fp64"] = pts["mixed"]["value"] / pts["fp64"]["value"]
+        out["accuracy_point"] = pts
+        del plan2
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(x, q, w, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(x, q, w, plan, args.cpu_budget)
     print(json.dumps(out))
 
 

@@ -827,8 +827,10 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
   }
   if (!by_level) G.cscale.clear();
   // global group per pair; by_level: a pair whose radii differ from its
-  // group's first pair runs per target (radius not uniform on its level);
-  // counts per (run, group) and per target of the ungrouped rest
+  // group's first pair (the chunks' scale) or from its reflection class's
+  // first pair (the tiles' operator scale, one per class) runs per target
+  // (radius not uniform on its level); counts per (run, group) and per
+  // target of the ungrouped rest
   std::vector<std::vector<int>> cnt(T);
   par([&](int r) {
     std::vector<int> &cn = cnt[r];
@@ -837,9 +839,13 @@ M2LGroups build_m2l_groups(const Tree &t, const Lists &L, int chunk_pairs, bool 
     for (int c = cut[r]; c < cut[r + 1]; c++)
       for (int p = L.m2l_ptr[c]; p < L.m2l_ptr[c + 1]; p++) {
         int g = lid[p] >= 0 ? m[lid[p]] : -1;
-        if (g >= 0 && by_level &&
-            (t.R[cell_of(L.m2l_src[p])] != sc[2 * (size_t)g] || t.R[c] != sc[2 * (size_t)g + 1]))
-          g = -1;
+        if (g >= 0 && by_level) {
+          const double rs = t.R[cell_of(L.m2l_src[p])], rt = t.R[c];
+          const size_t cg = (size_t)G.canon[g];
+          if (rs != sc[2 * (size_t)g] || rt != sc[2 * (size_t)g + 1] || rs != G.cscale[2 * cg] ||
+              rt != G.cscale[2 * cg + 1])
+            g = -1;
+        }
         lid[p] = g;
         if (g >= 0)
           cn[g]++;

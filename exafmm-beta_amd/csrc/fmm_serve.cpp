@@ -64,6 +64,7 @@ struct fmm_server {
   int64_t next_id = 1;
   int64_t inflight_bytes = 0;
   int inflight_batches = 0;
+  int waiters = 0;  // threads inside fmm_server_wait
   bool stop = false;
   std::vector<std::thread> builders, slots;
   fmm_server_stats stats{};
@@ -311,8 +312,11 @@ int fmm_server_wait(fmm_server *s, int64_t id, double *trg, fmm_request_info *in
     auto it = s->all.find(id);
     if (it == s->all.end()) return FMM_ERR_ARG;
     r = it->second;
+    s->waiters++;
     s->cv_done.wait(lk, [&] { return r->status == 3; });
-    s->all.erase(it);
+    s->waiters--;
+    s->all.erase(id);
+    if (s->stop) s->cv_done.notify_all();  // fmm_server_destroy waits for the last waiter
   }
   if (info) *info = r->info;
   if (r->rc != FMM_OK) return r->rc;
@@ -337,6 +341,19 @@ void fmm_server_destroy(fmm_server *s) {
   s->cv_ready.notify_all();
   for (auto &t : s->builders) t.join();
   for (auto &t : s->slots) t.join();
+  {
+    // requests that never ran end with FMM_ERR_STATE; waits blocked on them
+    // return before the server's mutex and condition variables go away
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (auto &kv : s->all)
+      if (kv.second->status != 3) {
+        kv.second->rc = FMM_ERR_STATE;
+        kv.second->err = "server destroyed before the request ran";
+        kv.second->status = 3;
+      }
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   for (auto &kv : s->ready) fmm_plan_destroy(kv.second->plan);
   delete s;
 }

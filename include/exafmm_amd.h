@@ -282,8 +282,10 @@ int fmm_server_submit(fmm_server *s, int n, const double *x, const double *q, in
  * info optional.  Each id is waited for once. */
 int fmm_server_wait(fmm_server *s, int64_t id, double *trg, fmm_request_info *info);
 int fmm_server_get_stats(fmm_server *s, fmm_server_stats *out);
-/* stops the threads after the batches in flight; call after the last wait
- * (requests still queued are dropped) */
+/* stops the threads after the batches in flight.  Requests that have not
+ * run end with FMM_ERR_STATE: a fmm_server_wait blocked on one returns that
+ * code, and destroy returns only after every such wait has returned.  No
+ * wait may start once destroy has been called. */
 void fmm_server_destroy(fmm_server *s);
 
 #ifdef __cplusplus

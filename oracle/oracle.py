@@ -14,6 +14,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libexafmm_ref.so")
+# the same reference sources built without -ffast-math (IEEE libm calls and no
+# reassociation): used to show that the generators differ only by that flag
+REF_IEEE_SO = os.path.join(HERE, "_ref", "libexafmm_ref_ieee.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -41,6 +44,7 @@ def _load(path):
 
 _lib = None
 _ref = None
+_ref_ieee = None
 
 
 def lib():
@@ -117,6 +121,9 @@ def ref():
         R.ref_init_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         R.ref_time.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip,
                                _dp, _dp, ctypes.c_long, _ip, _ip, ctypes.c_long, _ip, _ip, _dp, _dp]
+        R.ref_time_sampled.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip,
+                                       _ip, _dp, _dp, ctypes.c_int, ctypes.c_long, _ip, _ip,
+                                       ctypes.c_long, _ip, _ip, _dp, _dp]
         _ref = R
     return _ref
 
@@ -319,6 +326,25 @@ def ref_time(tree, ref_kernel, x_sorted, q_sorted, m2l_pairs, p2p_pairs):
     return tuple(times)
 
 
+def ref_time_sampled(ref_kernel, tree, x_sorted, q_sorted, stride, m2l_pairs, p2p_pairs):
+    """Time the reference operators on one core over every `stride`-th cell
+    of each pass and the given M2L / P2P pairs.  tree: dict of the CellBase
+    arrays (iparent, ichild, nchild, ibody, nbody, X, R).  Returns (seconds,
+    work) for P2M, M2M, M2L, P2P, L2L, L2P."""
+    R = ref()
+    xs, qs = _c(x_sorted), _c(q_sorted)
+    a = {k: _c(tree[k], np.int32) for k in ("iparent", "ichild", "nchild", "ibody", "nbody")}
+    mi, mj = (_c(v, np.int32) for v in m2l_pairs)
+    pi, pj = (_c(v, np.int32) for v in p2p_pairs)
+    times, work = np.zeros(6), np.zeros(6)
+    R.ref_time_sampled(ref_kernel.h, len(xs), _ptr(xs), _ptr(qs), len(a["nbody"]), _ptr(a["iparent"], _ip),
+                       _ptr(a["ichild"], _ip), _ptr(a["nchild"], _ip), _ptr(a["ibody"], _ip),
+                       _ptr(a["nbody"], _ip), _ptr(_c(tree["X"])), _ptr(_c(tree["R"])), int(stride),
+                       len(mi), _ptr(mi, _ip), _ptr(mj, _ip), len(pi), _ptr(pi, _ip), _ptr(pj, _ip),
+                       _ptr(times), _ptr(work))
+    return times, work
+
+
 def direct(kernel, xi, xj, qj):
     xi, xj, qj = _c(xi), _c(xj), _c(qj)
     trg = np.zeros((len(xi), 4))
@@ -388,8 +414,22 @@ def sample_indices(n, ntarget):
     return idx[:k]
 
 
-def ref_init_bodies(n, dist="cube", seed=0):
+def ref_ieee_available():
+    return os.path.exists(REF_IEEE_SO)
+
+
+def ref_init_bodies(n, dist="cube", seed=0, ieee=False):
+    """Dataset::initBodies of the compiled reference (include/dataset.h:230-255);
+    ieee=True: the build without -ffast-math."""
+    global _ref_ieee
+    if ieee:
+        if _ref_ieee is None:
+            _ref_ieee = _load(REF_IEEE_SO)
+            _ref_ieee.ref_init_bodies.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        R = _ref_ieee
+    else:
+        R = ref()
     x = np.zeros((n, 3))
     q = np.zeros(n)
-    ref().ref_init_bodies(n, dist.encode(), seed, _ptr(x), _ptr(q))
+    R.ref_init_bodies(n, dist.encode(), seed, _ptr(x), _ptr(q))
     return x, q

@@ -245,6 +245,84 @@ void ref_time(void *kp, int n, const double *x, const double *q, int ncell, cons
   times[3] = sec(t3, t4);
 }
 
+// Sampled variant for the CPU baseline of large systems: every `stride`-th
+// leaf's P2M, internal cell's M2M, non-root cell's L2L and leaf's L2P, and
+// the given M2L and P2P pairs, each phase timed separately on one core over
+// cells whose expansions hold what the earlier phases left (sampled: partial
+// sums, same operation count).  times[0..5] = P2M, M2M, M2L, P2P, L2L, L2P
+// seconds; work[0..5] = bodies, children, pairs, body pairs, cells, bodies
+// processed (for scaling to the whole solve).
+void ref_time_sampled(void *kp, int n, const double *x, const double *q, int ncell, const int *iparent,
+                      const int *ichild, const int *nchild, const int *ibody, const int *nbody,
+                      const double *X, const double *R, int stride, long nm2l, const int *mi,
+                      const int *mj, long np2p, const int *pi, const int *pj, double *times,
+                      double *work) {
+  using clk = std::chrono::steady_clock;
+  Kernel &K = static_cast<RefCtx *>(kp)->kernel;
+  Bodies bodies;
+  load_bodies(bodies, n, x, q);
+  Cells cells(ncell);
+  for (int c = 0; c < ncell; c++) {
+    set_cell(cells[c], X + 3 * c, R[c], K.NTERM);
+    cells[c].IPARENT = iparent[c];
+    cells[c].ICHILD = ichild[c];
+    cells[c].NCHILD = nchild[c];
+    cells[c].IBODY = ibody[c];
+    cells[c].NBODY = nbody[c];
+    cells[c].BODY = bodies.begin() + ibody[c];
+  }
+  for (int k = 0; k < 6; k++) times[k] = work[k] = 0;
+  if (ncell == 0 || stride < 1) return;
+  K.Xperiodic = 0;
+  C_iter C0 = cells.begin();
+  auto sec = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+  };
+  auto t0 = clk::now();
+  int k = 0;
+  for (int c = 0; c < ncell; c++)
+    if (nchild[c] == 0 && (k++ % stride) == 0) {
+      K.P2M(C0 + c);
+      work[0] += nbody[c];
+    }
+  auto t1 = clk::now();
+  k = 0;
+  for (int c = ncell - 1; c >= 0; c--)
+    if (nchild[c] > 0 && (k++ % stride) == 0) {
+      K.M2M(C0 + c, C0);
+      work[1] += nchild[c];
+    }
+  auto t2 = clk::now();
+  for (long p = 0; p < nm2l; p++) K.M2L(C0 + mi[p], C0 + mj[p]);
+  auto t3 = clk::now();
+  for (long p = 0; p < np2p; p++) {
+    K.P2P(C0 + pi[p], C0 + pj[p]);
+    work[3] += (double)nbody[pi[p]] * nbody[pj[p]];
+  }
+  auto t4 = clk::now();
+  k = 0;
+  for (int c = 1; c < ncell; c++)
+    if ((k++ % stride) == 0) {
+      K.L2L(C0 + c, C0);
+      work[4] += 1;
+    }
+  auto t5 = clk::now();
+  k = 0;
+  for (int c = 0; c < ncell; c++)
+    if (nchild[c] == 0 && (k++ % stride) == 0) {
+      K.L2P(C0 + c);
+      work[5] += nbody[c];
+    }
+  auto t6 = clk::now();
+  work[2] = (double)nm2l;
+  times[0] = sec(t0, t1);
+  times[1] = sec(t1, t2);
+  times[2] = sec(t2, t3);
+  times[3] = sec(t3, t4);
+  times[4] = sec(t4, t5);
+  times[5] = sec(t5, t6);
+}
+
 // Dataset::initBodies (include/dataset.h:230-255): positions + initSource.
 void ref_init_bodies(int n, const char *dist, int seed, double *x, double *q) {
   Dataset data;

@@ -1,0 +1,201 @@
+"""The benchmark configurations at their own sizes, on the GPU.
+
+  C1  N = 32768 uniform in [-1,1]^3, zero-mean q in [-1,1], P = 10, theta = 0.4,
+      ncrit = 64: every body against the oracle's FMM (1e-10) and the error
+      against the direct sum on 100 sampled targets.
+  C2  N = 2^20 Plummer (a = 1, r < 10), q = 1/N, P = 10: the oracle's FMM on
+      every body of 16 sampled target leaves (1e-10; the oracle's M2L, L2L and
+      L2P restricted to those leaves' lists and ancestors), the error against
+      the direct sum on 1000 sampled targets, and a repeated solve.
+  C3  N = 8,388,608 uniform cube, P = 8 and 12: FP64 and mixed precision,
+      each against the direct sum on 1000 targets, and mixed against FP64.
+  C4  64 requests drawn from the serving distribution (N = 2^U[12,18], cube or
+      Plummer, P = 8, mixed): the same alone and in the mix (1e-12), and the
+      error of sampled requests against the direct sum.
+  C5  the 98,304-site periodic water box (images = 3, P = 10, theta = 0.35):
+      two successive steps of fmm_md_evaluate against the device Ewald sum
+      (itself equal to the oracle's Ewald sum, tests/test_gpu_md.py).
+
+Error bounds against the direct sum are stated per configuration: the FMM's
+truncation error at that P and theta with a margin of about 2x over the
+measured value, so a wrong operator, list or precision path fails them.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FMM_TOL = 1e-10
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+
+
+def _sampled_error(F, f, x, ntarget):
+    """rel-L2 (pot, acc) against the GPU direct sum at stride-sampled tree-order targets."""
+    trg = f.targets(input_order=False)
+    perm = f.tree()["perm"]
+    idx = np.arange(ntarget) * (len(x) // ntarget)
+    d = f.direct(x[perm][idx])
+    return O.rel_l2(d, trg[idx])
+
+
+def test_c1_cube32k_p10():
+    F = _F()
+    n = 32768
+    x, q = F.make_bodies(n, "cube1", 0)
+    f = F.FMM(P=10)
+    f.set_bodies(x, q)
+    f.evaluate()
+    trg = f.targets(input_order=False)
+    err = _sampled_error(F, f, x, 100)
+    f.close()
+    t = O.Tree(x, 64)
+    ref = t.fmm(O.Kernel(10), x[t.perm], q[t.perm], 0.4, 5000)
+    print("C1: vs oracle %.2e %.2e; vs direct %.3e %.3e"
+          % (_rel(trg[:, 0], ref[:, 0]), _rel(trg[:, 1:], ref[:, 1:]), *err))
+    assert _rel(trg[:, 0], ref[:, 0]) < FMM_TOL and _rel(trg[:, 1:], ref[:, 1:]) < FMM_TOL
+    assert err[0] < 1e-5 and err[1] < 1e-5
+
+
+def test_c2_plummer1m_p10():
+    F = _F()
+    n, P = 1 << 20, 10
+    x, q = F.make_bodies(n, "plummer1", 0)
+    f = F.FMM(P=P)
+    f.set_bodies(x, q)
+    f.evaluate()
+    trg = f.targets(input_order=False)
+    f.evaluate()
+    again = f.targets(input_order=False)
+    err = _sampled_error(F, f, x, 1000)
+    f.close()
+    # grouped M2L adds parts of split tiles atomically: repeatable to rounding
+    assert _rel(again[:, 0], trg[:, 0]) < 1e-13 and _rel(again[:, 1:], trg[:, 1:]) < 1e-13
+    print("C2: vs direct %.3e %.3e" % err)
+    assert err[0] < 1.5e-6 and err[1] < 6e-6
+    # the oracle on 16 sampled target leaves: M of every cell, then only the
+    # M2L pairs into the leaves and their ancestors, the leaves' P2P pairs,
+    # L2L down every chain and L2P
+    t = O.Tree(x, 64)
+    xs, qs = x[t.perm], q[t.perm]
+    k = O.Kernel(P)
+    M = t.upward(k, xs, qs)
+    ci, cj, ty = t.traverse(0.4, 5000)
+    leaves = np.nonzero(t.nchild == 0)[0]
+    sample = leaves[np.linspace(0, len(leaves) - 1, 16).astype(int)]
+    chain = set()
+    for c in sample:
+        while True:
+            chain.add(int(c))
+            if c == 0:
+                break
+            c = t.iparent[c]
+    in_chain = np.isin(ci, np.fromiter(chain, np.int32))
+    sel = ((ty == 0) & in_chain) | ((ty == 1) & np.isin(ci, sample))
+    ref, Lc = t.eval_lists(k, xs, qs, ci[sel], cj[sel], ty[sel], M)
+    t.downward(k, xs, Lc, ref)
+    bodies = np.concatenate([np.arange(t.ibody[c], t.ibody[c] + t.nbody[c]) for c in sample])
+    ep, ea = _rel(trg[bodies, 0], ref[bodies, 0]), _rel(trg[bodies, 1:], ref[bodies, 1:])
+    print("C2: %d bodies of 16 leaves vs oracle %.2e %.2e" % (len(bodies), ep, ea))
+    assert ep < FMM_TOL and ea < FMM_TOL
+
+
+@pytest.mark.parametrize("P,bound", [(8, (5e-5, 1e-4)), (12, (3e-6, 5e-6))])
+def test_c3_cube8m_mixed_vs_fp64(P, bound):
+    F = _F()
+    n = 8388608
+    x, q = F.make_bodies(n, "cube1", 0)
+    plan = F.Plan(x, ncrit=64, theta=0.4, nspawn=5000)
+    out = {}
+    for name, prec in (("fp64", F.FMM_FP64), ("mixed", F.FMM_MIXED)):
+        f = F.FMM(P=P, precision=prec)
+        f.set_bodies_plan(plan, q)
+        f.evaluate()
+        out[name] = (f.targets(input_order=False), _sampled_error(F, f, x, 1000))
+        f.close()
+    (t64, e64), (t32, e32) = out["fp64"], out["mixed"]
+    dp, da = _rel(t32[:, 0], t64[:, 0]), _rel(t32[:, 1:], t64[:, 1:])
+    print("C3 P=%d: fp64 vs direct %.3e %.3e; mixed vs direct %.3e %.3e; mixed vs fp64 %.2e %.2e"
+          % (P, *e64, *e32, dp, da))
+    assert e64[0] < bound[0] and e64[1] < bound[1]
+    assert e32[0] < bound[0] and e32[1] < bound[1]
+    assert dp < 1e-6 and da < 5e-6
+
+
+def test_c4_serving_requests_alone_and_mixed():
+    F = _F()
+    rng = np.random.default_rng(0)
+    k = 64
+    sizes = np.floor(2.0 ** rng.uniform(12, 18, k)).astype(int)
+    kinds = rng.integers(0, 2, k)
+    systems = [F.make_bodies(int(m), "cube1" if c == 0 else "plummer1", 1000 + i)
+               for i, (m, c) in enumerate(zip(sizes, kinds))]
+
+    def serve(**cfg):
+        s = F.Server(P=8, precision=F.FMM_MIXED, **cfg)
+        ids = [s.submit(x, q) for x, q in systems]
+        res = [s.wait(i) for i in ids]
+        st = s.stats()
+        s.close()
+        return res, st
+
+    alone, st1 = serve(slots=1, batch_requests=1)
+    mix, st2 = serve(slots=3, batch_requests=64, batch_bodies=1 << 21, batch_delay_ms=200.0)
+    assert st1["max_batch_requests"] == 1 and st2["max_batch_requests"] > 1
+    worst = 0.0
+    for (ta, _), (tm, _) in zip(alone, mix):
+        worst = max(worst, _rel(tm[:, 0], ta[:, 0]), _rel(tm[:, 1:], ta[:, 1:]))
+    print("C4: %d requests, %d bodies; alone vs mix worst %.2e; batches %d (max %d requests)"
+          % (k, int(sizes.sum()), worst, st2["batches"], st2["max_batch_requests"]))
+    assert worst < 1e-12
+    for i in range(0, k, 8):
+        x, q = systems[i]
+        idx = np.arange(100) * (len(x) // 100)
+        d = O.direct(O.Kernel(8), x[idx], x, q)
+        e = O.rel_l2(d, mix[i][0][idx])
+        assert e[0] < 2e-4 and e[1] < 2e-4, (i, e)
+
+
+def test_c5_water_box_against_ewald():
+    sys.path.insert(0, ROOT)
+    import bench
+    F = _F()
+    w = bench.MD["md-water98k-p10"]
+    x, q = bench.water_box(w)
+    L = w["box"]
+    alpha = 3.9 / (L / 2)
+    ksize = int(math.ceil(2 * alpha * math.sqrt(math.log(1e7)) * L / (2 * math.pi)))
+    rng = np.random.default_rng(1)
+    fe = F.FMM(P=w["P"], images=w["images"], cycle=L)
+    f64 = F.FMM(P=w["P"], theta=w["theta"], images=w["images"], cycle=L)
+    f32 = F.FMM(P=w["P"], theta=w["theta"], images=w["images"], cycle=L, precision=F.FMM_MIXED)
+    for step in range(2):
+        pe, ge = fe.ewald(x, q, ksize, alpha, 0.25 / math.pi, L / 2)
+        p64, g64 = f64.md_evaluate(x, q)
+        p32, g32 = f32.md_evaluate(x, q)
+        ee = float(q @ pe)
+        e64 = (abs(q @ p64 - ee) / abs(ee), _rel(g64, ge))
+        e32 = (abs(q @ p32 - ee) / abs(ee), _rel(g32, ge))
+        d = (abs(q @ p32 - q @ p64) / abs(q @ p64), _rel(g32, g64))
+        print("C5 step %d: fp64 vs Ewald energy %.2e force %.2e; mixed %.2e %.2e; mixed vs fp64 %.2e %.2e"
+              % (step, *e64, *e32, *d))
+        assert e64[0] < 2e-5 and e64[1] < 2e-4
+        assert e32[0] < 2e-5 and e32[1] < 2e-4
+        assert d[0] < 1e-6 and d[1] < 1e-5
+        x = x + rng.normal(scale=w["sigma"] * L, size=x.shape)
+    for f in (fe, f64, f32):
+        f.close()

@@ -65,3 +65,59 @@ def test_mixed_single_leaf_and_empty():
     assert _rel(t32[:, 0], d[:, 0]) < POT_TOL and _rel(t32[:, 1:], d[:, 1:]) < ACC_TOL
     t0, _ = _solve(F, np.zeros((0, 3)), np.zeros(0), 8, F.FMM_MIXED)
     assert t0.shape == (0, 4)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_mixed_caller_tree_with_uneven_radii(mode):
+    """A caller-built tree whose cells on one level do not share a radius
+    (half the cells of every level 0.1% larger): the FP32 operators are scaled
+    by the radii of the pairs they were built for, so a pair whose radii differ
+    from its operator's must run per target in FP64.  Against the FP64 solve
+    of the same tree and lists, at the tolerances above."""
+    F = _F()
+    x, q = F.make_bodies(20000, "cube1", 3)
+    t = O.Tree(x, 64)
+    R = t.R.copy()
+    R[1::2] *= 1.001
+    tree = {"iparent": t.iparent, "ichild": t.ichild, "nchild": t.nchild, "ibody": t.ibody,
+            "nbody": t.nbody, "X": t.X, "R": R}
+    out = []
+    for prec in (F.FMM_FP64, F.FMM_MIXED):
+        f = F.FMM(P=8, precision=prec, m2l_mode=mode)
+        f.set_tree(x[t.perm], q[t.perm], tree)
+        f.evaluate()
+        out.append((f.targets(input_order=False), f.stats()))
+        f.close()
+    (t64, s64), (t32, s32) = out
+    assert s32["m2l_pairs"] == s64["m2l_pairs"] and s32["m2l_ungrouped"] > 0
+    dp, da = _rel(t32[:, 0], t64[:, 0]), _rel(t32[:, 1:], t64[:, 1:])
+    print("uneven radii, mode %d: ungrouped %d of %d; mixed-fp64 %.2e %.2e"
+          % (mode, s32["m2l_ungrouped"], s32["m2l_pairs"], dp, da))
+    assert dp < POT_TOL and da < ACC_TOL
+
+
+@pytest.mark.parametrize("case", ["ncrit512", "coincident300"])
+def test_mixed_leaves_above_256_bodies(case):
+    """Leaves of more than 64 lanes x 4 targets = 256 bodies (ncrit above 256,
+    or more than ncrit coincident bodies kept in one leaf at the depth cap)
+    take the P2P kernels' loop over target chunks."""
+    F = _F()
+    n = 30000
+    x, q = F.make_bodies(n, "cube1", 4)
+    ncrit, theta = 64, 0.4
+    if case == "ncrit512":
+        ncrit = 512
+    else:
+        n = 6000
+        x, q = x[:n].copy(), q[:n] - q[:n].mean()
+        x[1000:1300] = x[7]
+        theta = 1e-6  # every interaction P2P: the solve is the direct sum
+    t64, _ = _solve(F, x, q, 8, F.FMM_FP64, ncrit=ncrit, theta=theta)
+    t32, s = _solve(F, x, q, 8, F.FMM_MIXED, ncrit=ncrit, theta=theta)
+    idx = np.concatenate([np.arange(1000, 1300), np.arange(0, n, 97)])
+    k = O.Kernel(8)
+    d = O.direct(k, x[idx], x, q)
+    e64, e32 = O.rel_l2(d, t64[idx]), O.rel_l2(d, t32[idx])
+    print("%s: leaves %d; vs direct fp64 %.2e %.2e mixed %.2e %.2e" % (case, s["nleaf"], *e64, *e32))
+    assert e64[0] < (1e-4 if theta > 0.1 else 1e-13) and e64[1] < (1e-4 if theta > 0.1 else 1e-13)
+    assert e32[0] < e64[0] + POT_TOL and e32[1] < e64[1] + ACC_TOL

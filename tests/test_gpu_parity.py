@@ -89,9 +89,12 @@ def test_full_matches_oracle_orders(P, dist, n):
     f.close()
 
 
-@pytest.mark.parametrize("theta,ncrit,nspawn", [(0.6, 16, 5000), (0.35, 32, 1000), (0.25, 8, 100)])
-def test_full_matches_oracle_params(theta, ncrit, nspawn):
-    x, q = _inputs(3000, "plummer", seed=3)
+@pytest.mark.parametrize("theta,ncrit,nspawn,n", [(0.6, 16, 5000, 3000), (0.35, 32, 1000, 3000),
+                                                   (0.25, 8, 100, 3000),
+                                                   # leaves above 256 bodies: the P2P loop over target chunks
+                                                   (0.4, 512, 5000, 12000)])
+def test_full_matches_oracle_params(theta, ncrit, nspawn, n):
+    x, q = _inputs(n, "plummer", seed=3)
     f = _solve(x, q, 6, theta, ncrit, nspawn)
     trg = f.targets(input_order=False)
     t = O.Tree(x, ncrit)
@@ -175,6 +178,23 @@ def test_coincident_bodies_masked():
     t = O.Tree(x, 64)
     ref = t.fmm(O.Kernel(6), x[t.perm], q[t.perm], 0.4, 5000)
     assert _rel(trg, ref) < FMM_TOL
+    f.close()
+
+
+def test_coincident_leaf_above_256_bodies():
+    """301 coincident bodies stay in one leaf at the depth cap: the P2P
+    kernel's loop over target chunks, with R^2 = 0 pairs skipped.  theta -> 0
+    makes every interaction P2P, so the solve equals the direct sum (the
+    oracle's tree has no depth cap and cannot build this input)."""
+    x, q = _inputs(6000, seed=12)
+    x[1000:1300] = x[7]
+    f = _solve(x, q, 4, theta=1e-6)
+    assert f.stats()["m2l_pairs"] == 0
+    assert f.tree()["nbody"].max() >= 301
+    trg = f.targets(input_order=True)
+    d = O.direct(O.Kernel(4), x, x, q)
+    assert np.all(np.isfinite(trg))
+    assert _rel(trg[:, 0], d[:, 0]) < 1e-13 and _rel(trg[:, 1:], d[:, 1:]) < 1e-13
     f.close()
 
 

@@ -84,3 +84,30 @@ def test_empty_and_tiny_requests():
     t1, _ = s.wait(ids[1])
     s.close()
     assert t0.shape == (0, 4) and np.array_equal(t1, np.zeros((1, 4)))
+
+
+def test_destroy_releases_blocked_waits():
+    """Destroying the loop while requests are queued: a wait blocked on one
+    of them returns (the result, or FMM_ERR_STATE if it never ran) before
+    destroy returns, and destroy does not hang."""
+    import threading
+    import time
+    F = _F()
+    s = F.Server(P=8, slots=1, batch_requests=1)
+    systems = _systems(F, 6, 3) + [F.make_bodies(1 << 17, "cube1", 9)]
+    ids = [s.submit(x, q) for x, q in systems]
+    out = {}
+
+    def waiter():
+        try:
+            s.wait(ids[-1])
+            out["rc"] = 0
+        except F.FMMError as e:
+            out["rc"] = str(e)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    time.sleep(0.3)
+    s.close()
+    th.join(timeout=60)
+    assert not th.is_alive() and "rc" in out

@@ -59,6 +59,28 @@ def test_generators_match_reference():
         assert np.array_equal(O.data_source(3000, 5), q)
 
 
+@pytest.mark.skipif(not O.ref_ieee_available(), reason="oracle/_ref IEEE build absent")
+@pytest.mark.parametrize("n,seed", [(3000, 5), (20000, 0)])
+def test_plummer_matches_reference_up_to_fast_math(n, seed):
+    """orc_data_plummer (include/dataset.h:109-136) is bit-equal to the
+    reference's generator compiled without -ffast-math.  The shipped build
+    (-ffast-math) rounds the pow/sqrt/cos chain differently in the last bits
+    (at most a few 1e-13 relative); the octree and the interaction lists built
+    from either set are identical, so every count and error downstream is too."""
+    x = O.data_plummer(n, seed)
+    xi, qi = O.ref_init_bodies(n, "plummer", seed, ieee=True)
+    assert np.array_equal(x, xi) and np.array_equal(O.data_source(n, seed), qi)
+    xf, _ = O.ref_init_bodies(n, "plummer", seed)
+    rel = np.abs(xf - x) / np.maximum(np.abs(x), 1e-300)
+    assert rel.max() < 1e-12
+    t, tf = O.Tree(x, 64), O.Tree(xf, 64)
+    assert t.ncell == tf.ncell and np.array_equal(t.perm, tf.perm)
+    for k in ("iparent", "ichild", "nchild", "ibody", "nbody"):
+        assert np.array_equal(getattr(t, k), getattr(tf, k))
+    a, b = t.traverse(0.4, 5000), tf.traverse(0.4, 5000)
+    assert all(np.array_equal(u, v) for u, v in zip(a, b))
+
+
 # ---------------------------------------------------------------- golden numbers from the reference binary
 def test_golden_counts_cube32768():
     """examples/fmm.cxx defaults (cube, N=32768, ncrit=64, theta=.4): the reference
@@ -254,6 +276,189 @@ def test_tree_invariants():
         assert np.all(t.iparent[ch] == c)
         assert t.nbody[ch].sum() == t.nbody[c]
         assert np.allclose(t.R[ch], t.R[c] / 2)
+
+
+# ---------------------------------------------------------------- tree and traversal by invariants
+def _cell_of_bodies(t):
+    """(cell, body) index pairs for every cell and every body in its range."""
+    cells = np.repeat(np.arange(t.ncell), t.nbody)
+    starts = np.repeat(t.ibody, t.nbody)
+    offs = np.arange(len(cells)) - np.repeat(np.cumsum(t.nbody) - t.nbody, t.nbody)
+    return cells, starts + offs
+
+
+def check_tree(x, t, ncrit):
+    """What fixes BuildTree::buildTree's output (include/build_tree.h:29-209,
+    bound_box.h:11-25) without running it: the root box from the drifted
+    bounds, octant children at +-R/2, every body inside its cell's box and in
+    the octant of its child, children in increasing octant order and
+    contiguous, leaves at most ncrit bodies, and a stable split (so the bodies
+    of a leaf keep their input order): together they determine the
+    permutation and the cells."""
+    n = len(x)
+    xs = x[t.perm]
+    assert np.array_equal(np.sort(t.perm), np.arange(n))
+    lo, hi = O.bounds(x)
+    X0 = (lo + hi) / 2
+    R0 = max(np.max(X0 - lo), np.max(hi - X0)) * 1.00001
+    assert np.array_equal(t.X[0], X0) and t.R[0] == R0
+    assert t.ibody[0] == 0 and t.nbody[0] == n
+    leaf = t.nchild == 0
+    assert np.all(t.nbody[leaf] >= 1) and t.nbody[leaf].sum() == n
+    assert np.all(t.nbody[leaf] <= ncrit)
+    assert np.all(t.nbody[~leaf] > ncrit)
+    # every body inside the box of every cell that holds it
+    c, b = _cell_of_bodies(t)
+    assert np.all(np.abs(xs[b] - t.X[c]) <= t.R[c][:, None])
+    # children: contiguous block, parent link, half radius, centre at +-R/2, octants increasing
+    inner = np.nonzero(~leaf)[0]
+    for p in inner:
+        ch = np.arange(t.ichild[p], t.ichild[p] + t.nchild[p])
+        assert np.all(t.iparent[ch] == p) and np.all(t.R[ch] == t.R[p] / 2)
+        d = t.X[ch] - t.X[p]
+        assert np.allclose(np.abs(d), t.R[p] / 2, rtol=1e-14, atol=0)
+        octs = ((d > 0) * np.array([1, 2, 4])).sum(1)
+        assert np.all(np.diff(octs) > 0)
+        assert t.ibody[ch[0]] == t.ibody[p] and np.all(t.ibody[ch[1:]] == t.ibody[ch[:-1]] + t.nbody[ch[:-1]])
+        assert t.nbody[ch].sum() == t.nbody[p]
+        # each body of a child sits in that child's octant of the parent centre
+        for k, o in zip(ch, octs):
+            xb = xs[t.ibody[k]:t.ibody[k] + t.nbody[k]]
+            got = ((xb > t.X[p]) * np.array([1, 2, 4])).sum(1)
+            assert np.all(got == o)
+    # stable split: within a leaf the input order is kept
+    for k in np.nonzero(leaf)[0]:
+        assert np.all(np.diff(t.perm[t.ibody[k]:t.ibody[k] + t.nbody[k]]) > 0)
+    # cells numbered block-depth-first (nodes2cells): the children blocks of a
+    # cell's subtree follow its own children block in child order
+    nxt = [1]
+
+    def walk(cell):
+        if t.nchild[cell] == 0:
+            return
+        assert t.ichild[cell] == nxt[0]
+        nxt[0] += t.nchild[cell]
+        for k in range(t.ichild[cell], t.ichild[cell] + t.nchild[cell]):
+            walk(k)
+    import sys
+    sys.setrecursionlimit(10000)
+    walk(0)
+    assert nxt[0] == t.ncell
+
+
+def _mac(t, a, b, theta, xper):
+    """traversal.h:117-119: the pair is far enough for M2L."""
+    d = t.X[a] - t.X[b] - xper
+    return (d * d).sum(1) * theta * theta > (t.R[a] + t.R[b]) ** 2 * (1 - 1e-3)
+
+
+def check_lists(t, ci, cj, ty, theta, xper=np.zeros(3)):
+    """What fixes the dual traversal's lists (include/traversal.h:90-177)
+    without running it: every M2L pair passes the acceptance test and every
+    P2P pair is two leaves that fail it; every ordered (target, source) body
+    pair is covered exactly once, by a P2P pair of their leaves or an M2L pair
+    of an ancestor-or-self of each; and no pair is finer than needed: one of
+    the pairs it can have been split from fails the test."""
+    leaf = t.nchild == 0
+    m2l, p2p = ty == 0, ty == 1
+    assert np.all(m2l | p2p)
+    assert np.all(_mac(t, ci[m2l], cj[m2l], theta, xper))
+    assert np.all(leaf[ci[p2p]] & leaf[cj[p2p]])
+    assert not np.any(_mac(t, ci[p2p], cj[p2p], theta, xper))
+    # exact cover: per target leaf, the source body ranges tile [0, n)
+    nc = t.ncell
+    order = np.argsort(ci, kind="stable")
+    ptr = np.zeros(nc + 1, np.int64)
+    ptr[1:] = np.cumsum(np.bincount(ci, minlength=nc))
+    src = cj[order]
+    for k in np.nonzero(leaf)[0]:
+        chain, a = [k], k
+        while a != 0:
+            a = t.iparent[a]
+            chain.append(a)
+        rows = [src[ptr[a]:ptr[a + 1]] for a in chain]
+        # P2P sources only on the leaf itself; ancestors hold M2L sources only
+        s = np.concatenate(rows)
+        st, ln = t.ibody[s], t.nbody[s]
+        o = np.argsort(st, kind="stable")
+        st, ln = st[o], ln[o]
+        assert st[0] == 0 and np.all(st[1:] == st[:-1] + ln[:-1]) and st[-1] + ln[-1] == t.n, k
+        for a in chain[1:]:
+            assert np.all(ty[order[ptr[a]:ptr[a + 1]]] == 0)
+    # not finer than needed
+    root = (ci == 0) & (cj == 0)
+    a, b = ci[~root], cj[~root]
+    pa, pb = t.iparent[a], t.iparent[b]
+    has_pa, has_pb = a != 0, b != 0
+    fail_a = has_pa & ~_mac(t, np.where(has_pa, pa, 0), b, theta, xper)
+    fail_b = has_pb & ~_mac(t, a, np.where(has_pb, pb, 0), theta, xper)
+    fail_ab = has_pa & has_pb & ~_mac(t, np.where(has_pa, pa, 0), np.where(has_pb, pb, 0), theta, xper)
+    assert np.all(fail_a | fail_b | fail_ab)
+
+
+@pytest.mark.parametrize("dist,n,ncrit", [("cube", 20000, 64), ("plummer", 15000, 16), ("cube", 3000, 4),
+                                          ("plummer", 40000, 64)])
+def test_tree_fixed_by_invariants(dist, n, ncrit):
+    x = O.data_cube(n, 7) if dist == "cube" else O.data_plummer(n, 7)
+    check_tree(x, O.Tree(x, ncrit), ncrit)
+
+
+@pytest.mark.parametrize("dist,n,ncrit,theta,nspawn,shift", [
+    ("cube", 20000, 64, 0.4, 5000, 0), ("plummer", 15000, 16, 0.3, 5000, 0), ("cube", 8000, 32, 0.6, 100, 0),
+    ("plummer", 10000, 8, 0.5, 1, 0), ("cube", 12000, 64, 0.35, 5000, 0), ("cube", 6000, 32, 0.4, 5000, 1),
+    ("plummer", 30000, 64, 0.4, 5000, 0)])
+def test_lists_fixed_by_invariants(dist, n, ncrit, theta, nspawn, shift):
+    x = O.data_cube(n, 3) if dist == "cube" else O.data_plummer(n, 3)
+    t = O.Tree(x, ncrit)
+    xper = np.array([2 * math.pi, 0.0, -2 * math.pi]) * shift
+    ci, cj, ty = t.traverse(theta, nspawn, xper)
+    check_lists(t, ci, cj, ty, theta, xper)
+
+
+@pytest.mark.parametrize("sign", [1, -1])
+def test_acceptance_test_known_answer(sign):
+    """Two bodies at opposite corners, ncrit = 1: the root splits into two
+    leaves of radius R0/2 whose centres differ by R0 along every axis, so
+    |dX|^2 = 3 R0^2 and (Ri + Rj)^2 = R0^2.  The acceptance test
+    |dX|^2 theta^2 > (Ri + Rj)^2 (1 - 1e-3) (traversal.h:117-119) flips at
+    theta^2 = (1 - 1e-3) / 3: just above it the two cross pairs are M2L, just
+    below they are P2P; between that and 1/3 (where a test without the 1e-3
+    margin would flip) they are still M2L."""
+    x = np.array([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])
+    t = O.Tree(x, 1)
+    assert t.ncell == 3 and np.array_equal(t.X[2] - t.X[1], np.full(3, t.R[0]))
+    th = math.sqrt((1 - 1e-3) / 3) * (1 + sign * 1e-9)
+    ci, cj, ty = t.traverse(th, 5000)
+    assert len(ty) == 4 and sorted(zip(ci, cj)) == [(1, 1), (1, 2), (2, 1), (2, 2)]
+    cross = ci != cj
+    assert np.all(ty[~cross] == 1)
+    assert np.all(ty[cross] == (0 if sign > 0 else 1))
+    _, _, ty = t.traverse(math.sqrt(1 / 3) * (1 - 1e-5), 5000)
+    assert int((ty == 0).sum()) == 2
+
+
+def test_invariant_checks_catch_planted_mistakes():
+    """The checks above are not vacuous: lists with a wrong acceptance factor,
+    a dropped pair or a duplicated pair, and a tree with two bodies swapped,
+    each fail them."""
+    x = O.data_cube(6000, 4)
+    t = O.Tree(x, 32)
+    ci, cj, ty = t.traverse(0.4, 5000)
+    check_lists(t, ci, cj, ty, 0.4)
+    bad = [
+        t.traverse(0.4 * 1.1, 5000),                           # looser acceptance
+        (np.delete(ci, 100), np.delete(cj, 100), np.delete(ty, 100)),   # dropped pair
+        (np.append(ci, ci[7]), np.append(cj, cj[7]), np.append(ty, ty[7])),  # duplicated pair
+    ]
+    for lists in bad:
+        with pytest.raises(AssertionError):
+            check_lists(t, *lists, 0.4)
+    check_tree(x, t, 32)
+    leaf = np.nonzero(t.nchild == 0)[0][3]
+    i = t.ibody[leaf]
+    t.perm[[i, i + 1]] = t.perm[[i + 1, i]]
+    with pytest.raises(AssertionError):
+        check_tree(x, t, 32)
 
 
 # ---------------------------------------------------------------- periodic images
