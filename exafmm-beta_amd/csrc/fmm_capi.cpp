@@ -97,6 +97,7 @@ struct fmm_ctx {
   DBuf g_sc, cr, M32;
   DBuf dip;  // dipole partial sums
   DBuf ew_b, ew_t, ew_k, ew_s;  // fmm_ewald: bodies, targets, wave vectors, structure factors
+  DBuf op_d, op_i;  // fmm_kernel_*: doubles (bodies, centres, expansions) and ints of one call
 };
 
 struct fmm_plan {
@@ -110,6 +111,7 @@ struct fmm_plan {
   fmm::M2LTiles tiles;    // prebuilt target tiles of those groups (TT > 0)
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   double t_bounds = 0, t_tree = 0, t_lists = 0;
+  int pair_targets = -1;  // fmm_plan_pair: bodies [0, pair_targets) are the targets
 };
 
 #define FMM_FAIL(code, msg) \
@@ -344,6 +346,72 @@ int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double the
 }
 
 void fmm_plan_destroy(fmm_plan *pl) { delete pl; }
+
+int fmm_plan_build_tree(fmm_plan **out, int n, const double *x, int ncrit, const double *xmin,
+                        const double *xmax) {
+  if (!out) return FMM_ERR_ARG;
+  *out = nullptr;
+  if (n < 0 || (n > 0 && (!x || !xmin || !xmax)) || ncrit < 1) return FMM_ERR_ARG;
+  fmm_plan *pl = new (std::nothrow) fmm_plan();
+  if (!pl) return FMM_ERR_MEMORY;
+  try {
+    pl->n = n;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (n > 0)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = xmin[d];
+        hi[d] = xmax[d];
+      }
+    for (int d = 0; d < 3; d++) {
+      pl->xmin[d] = lo[d];
+      pl->xmax[d] = hi[d];
+    }
+    const double t0 = now_ms();
+    pl->tree = fmm::build_tree(n, x, ncrit, lo, hi);
+    pl->lists = fmm::empty_lists(pl->tree.ncell);
+    pl->t_tree = now_ms() - t0;
+  } catch (const std::bad_alloc &) {
+    delete pl;
+    return FMM_ERR_MEMORY;
+  }
+  *out = pl;
+  return FMM_OK;
+}
+
+int fmm_plan_pair(fmm_plan **out, const fmm_plan *targets, const fmm_plan *sources, double theta, int nspawn) {
+  if (!out) return FMM_ERR_ARG;
+  *out = nullptr;
+  if (!targets || !sources || !(theta > 0) || nspawn < 1) return FMM_ERR_ARG;
+  if (targets->images != 0 || sources->images != 0) return FMM_ERR_UNSUPPORTED;
+  if ((int64_t)targets->n + sources->n > INT32_MAX || targets->tree.ncell + (int64_t)sources->tree.ncell > fmm::kCellMask)
+    return FMM_ERR_ARG;
+  fmm_plan *pl = new (std::nothrow) fmm_plan();
+  if (!pl) return FMM_ERR_MEMORY;
+  try {
+    const fmm::Lists lt = fmm::empty_lists(targets->tree.ncell), ls = fmm::empty_lists(sources->tree.ncell);
+    std::vector<fmm::PlanPart> parts = {{&targets->tree, &lt, nullptr, nullptr},
+                                        {&sources->tree, &ls, nullptr, nullptr}};
+    fmm::merge_plans(parts, pl->tree, pl->lists, nullptr, fmmdev::kM2LChunkPairs);
+    const double t0 = now_ms();
+    const double xper[3] = {0, 0, 0};
+    if (targets->tree.ncell > 0 && sources->tree.ncell > 0)
+      pl->lists = fmm::build_lists(pl->tree, theta, nspawn, xper, 0, nullptr, 0, targets->tree.ncell);
+    pl->t_lists = now_ms() - t0;
+  } catch (const std::bad_alloc &) {
+    delete pl;
+    return FMM_ERR_MEMORY;
+  }
+  pl->n = targets->n + sources->n;
+  pl->pair_targets = targets->n;
+  for (int d = 0; d < 3; d++) {
+    pl->xmin[d] = targets->xmin[d];
+    pl->xmax[d] = targets->xmax[d];
+  }
+  pl->t_bounds = targets->t_bounds + sources->t_bounds;
+  pl->t_tree = targets->t_tree + sources->t_tree;
+  *out = pl;
+  return FMM_OK;
+}
 
 int fmm_plan_build_groups(fmm_plan *pl, int precision) {
   if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED)) return FMM_ERR_ARG;
@@ -1213,6 +1281,323 @@ int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p
   if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies");
   copy_lists(c->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);
   return FMM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- Kernel operators
+// The reference's per-cell Kernel methods (kernels/laplace.h:150-388) on
+// caller-held cells: each call lays its cells out as a small cell table on the
+// device (one int block, one double block), runs the same kernel the passes
+// use, and adds the result to the caller's arrays in the reference's
+// normalisation (M = M~ s, L = L^ / s, s = sqrt((n-m)!(n+m)!)).
+namespace {
+struct OpLayout {
+  // host staging: doubles, then ints; offsets in elements
+  std::vector<double> d;
+  std::vector<int> i;
+  size_t put(const double *v, size_t k) {
+    const size_t o = d.size();
+    d.insert(d.end(), v, v + k);
+    return o;
+  }
+  size_t zeros(size_t k) {
+    const size_t o = d.size();
+    d.resize(o + k, 0.0);
+    return o;
+  }
+  void align() {  // next double at a 16-byte offset (complex values)
+    if (d.size() & 1) d.push_back(0.0);
+  }
+  size_t puti(std::initializer_list<int> v) {
+    const size_t o = i.size();
+    i.insert(i.end(), v);
+    return o;
+  }
+  size_t zerosi(size_t k) {
+    const size_t o = i.size();
+    i.resize(o + k, 0);
+    return o;
+  }
+};
+
+struct OpDev {
+  double *d;
+  int *i;
+};
+
+int op_begin(fmm_ctx *c) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  HIPCHK(hipSetDevice(c->p.device));
+  return FMM_OK;
+}
+
+int op_upload(fmm_ctx *c, const OpLayout &h, OpDev &dv) {
+  int rc;
+  if ((rc = upload(c, c->op_d, h.d.data(), h.d.size()))) return rc;
+  if ((rc = upload(c, c->op_i, h.i.data(), h.i.size()))) return rc;
+  dv.d = c->op_d.as<double>();
+  dv.i = c->op_i.as<int>();
+  return FMM_OK;
+}
+
+int op_download(fmm_ctx *c, const OpDev &dv, size_t off, size_t count, std::vector<double> &out) {
+  out.resize(count);
+  if (count) HIPCHK(hipMemcpyAsync(out.data(), dv.d + off, sizeof(double) * count, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return FMM_OK;
+}
+
+std::vector<double> norm_table(int P) {  // s(n,m) at index n(n+1)/2+m
+  std::vector<double> s(P * (P + 1) / 2);
+  for (int n = 0; n < P; n++)
+    for (int m = 0; m <= n; m++) s[n * (n + 1) / 2 + m] = s_nm(n, m);
+  return s;
+}
+
+// cell table of k cells (SoA centres from X (k x 3)), bodies/children fields zero
+struct OpCells {
+  size_t cx, cy, cz, ibody, nbody, ichild, nchild, iparent;
+};
+OpCells op_cells(OpLayout &h, int k, const double *X) {
+  OpCells o;
+  o.cx = h.zeros(k);
+  o.cy = h.zeros(k);
+  o.cz = h.zeros(k);
+  for (int c = 0; c < k; c++) {
+    h.d[o.cx + c] = X[3 * (size_t)c];
+    h.d[o.cy + c] = X[3 * (size_t)c + 1];
+    h.d[o.cz + c] = X[3 * (size_t)c + 2];
+  }
+  o.ibody = h.zerosi(k);
+  o.nbody = h.zerosi(k);
+  o.ichild = h.zerosi(k);
+  o.nchild = h.zerosi(k);
+  o.iparent = h.zerosi(k);
+  return o;
+}
+fmmdev::DevCells dev_cells_of(const OpDev &dv, const OpCells &o) {
+  return {dv.d + o.cx, dv.d + o.cy, dv.d + o.cz, dv.i + o.ibody,
+          dv.i + o.nbody, dv.i + o.ichild, dv.i + o.nchild, dv.i + o.iparent};
+}
+// n bodies (SoA) from x (n x 3) and q (may be NULL), targets zeroed
+struct OpBodies {
+  size_t x, y, z, q, pot, ax, ay, az;
+};
+OpBodies op_bodies(OpLayout &h, int n, const double *x, const double *q) {
+  OpBodies o;
+  o.x = h.zeros(n);
+  o.y = h.zeros(n);
+  o.z = h.zeros(n);
+  o.q = h.zeros(n);
+  for (int b = 0; b < n; b++) {
+    h.d[o.x + b] = x[3 * (size_t)b];
+    h.d[o.y + b] = x[3 * (size_t)b + 1];
+    h.d[o.z + b] = x[3 * (size_t)b + 2];
+    if (q) h.d[o.q + b] = q[b];
+  }
+  o.pot = h.zeros(n);
+  o.ax = h.zeros(n);
+  o.ay = h.zeros(n);
+  o.az = h.zeros(n);
+  return o;
+}
+fmmdev::DevBodies dev_bodies_of(const OpDev &dv, const OpBodies &o) {
+  return {dv.d + o.x, dv.d + o.y, dv.d + o.z, dv.d + o.q, dv.d + o.pot, dv.d + o.ax, dv.d + o.ay, dv.d + o.az};
+}
+// the targets (pot, ax, ay, az) of n bodies added into trg (n x 4)
+int op_add_targets(fmm_ctx *c, const OpDev &dv, const OpBodies &o, int n, double *trg) {
+  std::vector<double> h;
+  int rc = op_download(c, dv, o.pot, 4 * (size_t)n, h);  // pot, ax, ay, az are contiguous
+  if (rc) return rc;
+  for (int b = 0; b < n; b++)
+    for (int k = 0; k < 4; k++) trg[4 * (size_t)b + k] += h[(size_t)k * n + b];
+  return FMM_OK;
+}
+// expansions: reference values -> device normalisation (M / s or L * s)
+size_t op_expansion(OpLayout &h, int P, const double *E, bool local) {
+  const std::vector<double> s = norm_table(P);
+  h.align();
+  const size_t nt = s.size(), o = h.zeros(2 * nt);
+  if (E)
+    for (size_t u = 0; u < nt; u++) {
+      const double f = local ? s[u] : 1.0 / s[u];
+      h.d[o + 2 * u] = E[2 * u] * f;
+      h.d[o + 2 * u + 1] = E[2 * u + 1] * f;
+    }
+  return o;
+}
+// device expansion at off added into E in the reference normalisation
+int op_add_expansion(fmm_ctx *c, const OpDev &dv, size_t off, int P, bool local, double *E) {
+  const std::vector<double> s = norm_table(P);
+  std::vector<double> h;
+  int rc = op_download(c, dv, off, 2 * s.size(), h);
+  if (rc) return rc;
+  for (size_t u = 0; u < s.size(); u++) {
+    const double f = local ? 1.0 / s[u] : s[u];
+    E[2 * u] += h[2 * u] * f;
+    E[2 * u + 1] += h[2 * u + 1] * f;
+  }
+  return FMM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fmm_kernel_p2p(fmm_ctx *c, int ni, const double *xi, double *trgi, int nj, const double *xj,
+                   const double *qj, const double *xper) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (ni < 0 || nj < 0 || (ni > 0 && (!xi || !trgi)) || (nj > 0 && (!xj || !qj)))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_p2p: bad sizes or NULL array");
+  if (ni == 0 || nj == 0) return FMM_OK;
+  if ((int64_t)ni + nj > INT32_MAX) FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_p2p: too many bodies");
+  OpLayout h;
+  // bodies: the ni targets, then the nj sources; cell 0 holds the targets, cell 1 the sources
+  std::vector<double> xb(3 * ((size_t)ni + nj)), qb((size_t)ni + nj, 0.0);
+  std::memcpy(xb.data(), xi, sizeof(double) * 3 * ni);
+  std::memcpy(xb.data() + 3 * (size_t)ni, xj, sizeof(double) * 3 * nj);
+  std::memcpy(qb.data() + ni, qj, sizeof(double) * nj);
+  const OpBodies B = op_bodies(h, ni + nj, xb.data(), qb.data());
+  const double X[6] = {0, 0, 0, 0, 0, 0};
+  const OpCells C = op_cells(h, 2, X);
+  h.i[C.ibody + 1] = ni;
+  h.i[C.nbody] = ni;
+  h.i[C.nbody + 1] = nj;
+  // list: target cell 0 <- source cell 1, shifted by xper through image 26
+  // (shift (1, 1, 1) * cycle with cycle = xper)
+  const size_t tg = h.puti({0}), ptr = h.puti({0, 1, 1});
+  const size_t src = h.puti({xper ? 1 | (26 << 26) : 1});
+  const double3 cyc = xper ? make_double3(xper[0], xper[1], xper[2]) : make_double3(0, 0, 0);
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_p2p(dev_bodies_of(dv, B), dev_cells_of(dv, C), dv.i + tg, 1, dv.i + ptr, dv.i + src, cyc,
+                            c->p.eps2, 1, ni >= fmmdev::kP2PWideLeaf ? 1 : 0, c->st));
+  return op_add_targets(c, dv, B, ni, trgi);
+}
+
+int fmm_kernel_p2m(fmm_ctx *c, const double *Xc, int n, const double *x, const double *q, double *M) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (!Xc || !M || n < 0 || (n > 0 && (!x || !q))) FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_p2m: bad arguments");
+  if (n == 0) return FMM_OK;
+  OpLayout h;
+  const OpBodies B = op_bodies(h, n, x, q);
+  const OpCells C = op_cells(h, 1, Xc);
+  h.i[C.nbody] = n;
+  const size_t leaves = h.puti({0});
+  h.align();
+  const size_t Mo = h.zeros(2 * (size_t)nterm(c));
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_p2m(c->p.P, dev_bodies_of(dv, B), dev_cells_of(dv, C), dv.i + leaves, 1,
+                            reinterpret_cast<double2 *>(dv.d + Mo), c->st));
+  return op_add_expansion(c, dv, Mo, c->p.P, false, M);
+}
+
+int fmm_kernel_m2m(fmm_ctx *c, const double *Xp, int nchild, const double *Xc, const double *Mc, double *Mp) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (!Xp || !Mp || nchild < 0 || (nchild > 0 && (!Xc || !Mc)) || nchild > (1 << 20))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_m2m: bad arguments");
+  if (nchild == 0) return FMM_OK;
+  const int NT = nterm(c);
+  OpLayout h;
+  std::vector<double> X(3 * ((size_t)nchild + 1));
+  std::memcpy(X.data(), Xp, sizeof(double) * 3);
+  std::memcpy(X.data() + 3, Xc, sizeof(double) * 3 * nchild);
+  const OpCells C = op_cells(h, nchild + 1, X.data());
+  h.i[C.ichild] = 1;
+  h.i[C.nchild] = nchild;
+  // multipoles of the parent (cell 0, overwritten) and the children
+  h.align();
+  const size_t Mo = h.zeros(2 * (size_t)NT);
+  for (int k = 0; k < nchild; k++) op_expansion(h, c->p.P, Mc + 2 * (size_t)NT * k, false);
+  const size_t cells = h.puti({0});
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_m2m(c->p.P, dev_cells_of(dv, C), dv.i + cells, 1, reinterpret_cast<double2 *>(dv.d + Mo),
+                            c->st));
+  return op_add_expansion(c, dv, Mo, c->p.P, false, Mp);
+}
+
+int fmm_kernel_m2l(fmm_ctx *c, int npair, const double *Xi, const double *Xj, const double *xper,
+                   const double *Mj, double *Li) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (npair < 0 || (npair > 0 && (!Xi || !Xj || !Mj || !Li)) || npair > (1 << 24))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_m2l: bad arguments");
+  if (npair == 0) return FMM_OK;
+  const int NT = nterm(c);
+  OpLayout h;
+  // cells [0, npair) the targets, [npair, 2 npair) the sources
+  std::vector<double> X(6 * (size_t)npair);
+  std::memcpy(X.data(), Xi, sizeof(double) * 3 * npair);
+  std::memcpy(X.data() + 3 * (size_t)npair, Xj, sizeof(double) * 3 * npair);
+  const OpCells C = op_cells(h, 2 * npair, X.data());
+  h.align();
+  const size_t Mo = h.zeros(2 * (size_t)NT * npair);  // targets' M (unused)
+  for (int k = 0; k < npair; k++) op_expansion(h, c->p.P, Mj + 2 * (size_t)NT * k, false);
+  h.align();
+  const size_t Lo = h.zeros(4 * (size_t)NT * npair);
+  const size_t tg = h.zerosi(npair), ptr = h.zerosi(npair + 1), src = h.zerosi(npair);
+  for (int k = 0; k < npair; k++) {
+    h.i[tg + k] = k;
+    h.i[ptr + k + 1] = k + 1;
+    h.i[src + k] = xper ? (npair + k) | (26 << 26) : npair + k;
+  }
+  const double3 cyc = xper ? make_double3(xper[0], xper[1], xper[2]) : make_double3(0, 0, 0);
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_m2l(c->p.P, dev_cells_of(dv, C), dv.i + tg, npair, dv.i + ptr, dv.i + src, cyc,
+                            reinterpret_cast<const double2 *>(dv.d + Mo), reinterpret_cast<double2 *>(dv.d + Lo), 0,
+                            c->st));
+  const std::vector<double> s = norm_table(c->p.P);
+  std::vector<double> o;
+  if ((rc = op_download(c, dv, Lo, 2 * (size_t)NT * npair, o))) return rc;
+  for (int k = 0; k < npair; k++)
+    for (int u = 0; u < NT; u++) {
+      Li[2 * ((size_t)NT * k + u)] += o[2 * ((size_t)NT * k + u)] / s[u];
+      Li[2 * ((size_t)NT * k + u) + 1] += o[2 * ((size_t)NT * k + u) + 1] / s[u];
+    }
+  return FMM_OK;
+}
+
+int fmm_kernel_l2l(fmm_ctx *c, const double *Xc, const double *Xp, const double *Lp, double *Lc) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (!Xc || !Xp || !Lp || !Lc) FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_l2l: NULL argument");
+  OpLayout h;
+  const double X[6] = {Xp[0], Xp[1], Xp[2], Xc[0], Xc[1], Xc[2]};
+  const OpCells C = op_cells(h, 2, X);  // cell 1 (the child) pulls from its parent, cell 0
+  const size_t Lo = op_expansion(h, c->p.P, Lp, true);
+  const size_t Lco = op_expansion(h, c->p.P, nullptr, true);
+  (void)Lco;  // follows the parent's block: cell 1's local expansion
+  const size_t cells = h.puti({1});
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_l2l(c->p.P, dev_cells_of(dv, C), dv.i + cells, 1, reinterpret_cast<double2 *>(dv.d + Lo),
+                            c->st));
+  return op_add_expansion(c, dv, Lo + 2 * (size_t)nterm(c), c->p.P, true, Lc);
+}
+
+int fmm_kernel_l2p(fmm_ctx *c, const double *Xc, const double *L, int n, const double *x, double *trg) {
+  int rc = op_begin(c);
+  if (rc) return rc;
+  if (!Xc || !L || n < 0 || (n > 0 && (!x || !trg))) FMM_FAIL(FMM_ERR_ARG, "fmm_kernel_l2p: bad arguments");
+  if (n == 0) return FMM_OK;
+  OpLayout h;
+  const OpBodies B = op_bodies(h, n, x, nullptr);
+  const OpCells C = op_cells(h, 1, Xc);
+  h.i[C.nbody] = n;
+  const size_t Lo = op_expansion(h, c->p.P, L, true);
+  const size_t leaf = h.zerosi(n);  // every body in cell 0
+  OpDev dv;
+  if ((rc = op_upload(c, h, dv))) return rc;
+  HIPCHK(fmmdev::launch_l2p(c->p.P, dev_bodies_of(dv, B), dev_cells_of(dv, C), dv.i + leaf, n,
+                            reinterpret_cast<const double2 *>(dv.d + Lo), c->st));
+  return op_add_targets(c, dv, B, n, trg);
 }
 
 }  // extern "C"

@@ -488,10 +488,16 @@ int host_threads() {
   return std::max(1, std::min(16, hw));
 }
 
-Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3], int images, const double *cycle) {
+Lists empty_lists(int ncell) {
   Lists L;
-  L.m2l_ptr.assign(t.ncell + 1, 0);
-  L.p2p_ptr.assign(t.ncell + 1, 0);
+  L.m2l_ptr.assign(ncell + 1, 0);
+  L.p2p_ptr.assign(ncell + 1, 0);
+  return L;
+}
+
+Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3], int images, const double *cycle,
+                  int root_i, int root_j) {
+  Lists L = empty_lists(t.ncell);
   if (t.ncell == 0) return L;
   const int nth = t.n >= 50000 ? host_threads() : 1;
   std::deque<Walker> ws;  // segments in emission order (stable addresses)
@@ -514,7 +520,7 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3],
       w.m2l_j.reserve(64 * (size_t)t.ncell / nimg);
       w.p2p_i.reserve(32 * (size_t)t.leaves.size() / nimg);
       w.p2p_j.reserve(32 * (size_t)t.leaves.size() / nimg);
-      w.dual(0, 0);
+      w.dual(root_i, root_j);
     }
   } else {
     // For every image, the recursion down to cell pairs of at most `cut`
@@ -530,7 +536,7 @@ Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3],
       Walker top{t, theta, nspawn, {xp[0], xp[1], xp[2]}, {}, {}, {}, {}};
       top.cut = std::max(2 * 64, t.n / (8 * nth));
       top.tasks = &tasks[img];
-      top.dual(0, 0);
+      top.dual(root_i, root_j);
       ntask += tasks[img].size();
     }
     struct Blk {

@@ -92,8 +92,13 @@ inline void image_shift(int img, const double cycle[3], double out[3]) {
 // dual traversal of the root with itself for each of the 27 shifts in the
 // reference's loop order, pairs tagged with their image (the hierarchical
 // far images of traversePeriodic are not pairs; see the periodic operator).
+// root_i, root_j: the pair the traversal starts from (0, 0 for one tree;
+// the two roots of a forest for traverse(icells, jcells), traversal.h:326-333)
 Lists build_lists(const Tree &t, double theta, int nspawn, const double xper[3], int images = 0,
-                  const double *cycle = nullptr);
+                  const double *cycle = nullptr, int root_i = 0, int root_j = 0);
+
+// Empty lists for a tree of ncell cells (a tree whose lists are made later).
+Lists empty_lists(int ncell);
 
 // M2L pairs grouped by their exact offset X_t - X_s.  Offsets between cells of
 // the reference's octree are integer multiples of R_root / 2^e; a group key is

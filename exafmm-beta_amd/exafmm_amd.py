@@ -29,6 +29,8 @@ SYMBOLS = [
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
     "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
     "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts", "fmm_ewald",
+    "fmm_plan_build_tree", "fmm_plan_pair", "fmm_kernel_p2p", "fmm_kernel_p2m", "fmm_kernel_m2m",
+    "fmm_kernel_m2l", "fmm_kernel_l2l", "fmm_kernel_l2p",
 ]
 
 FMM_FP64 = 0
@@ -155,6 +157,14 @@ def load():
         L.fmm_server_destroy.argtypes = [vp]
         L.fmm_server_destroy.restype = None
         L.fmm_plan_merge.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.fmm_plan_build_tree.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.fmm_plan_pair.argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_double, ctypes.c_int]
+        L.fmm_kernel_p2p.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.fmm_kernel_p2m.argtypes = [vp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.fmm_kernel_m2m.argtypes = [vp, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.fmm_kernel_m2l.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.fmm_kernel_l2l.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.fmm_kernel_l2p.argtypes = [vp, _dp, _dp, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -366,6 +376,70 @@ class FMM:
         return key[:nc], p2p[:nc], m2l[:nc]
 
 
+    # ---- Kernel operators (kernels/laplace.h:150-388) on caller-held cells:
+    # expansions are complex arrays of NTERM (reference normalisation); each
+    # call returns the input accumulator plus the operator's result
+    def _c3(self, v):
+        return np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+
+    def _cx(self, E, shape):
+        E = np.zeros(shape, np.complex128) if E is None else np.array(E, dtype=np.complex128, order="C")
+        return E
+
+    def kernel_p2p(self, xi, xj, qj, xper=None, trg=None):
+        xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 3)
+        xj = np.ascontiguousarray(xj, dtype=np.float64).reshape(-1, 3)
+        qj = np.ascontiguousarray(qj, dtype=np.float64).reshape(-1)
+        trg = np.zeros((len(xi), 4)) if trg is None else np.array(trg, dtype=np.float64, order="C")
+        xp = None if xper is None else self._c3(xper)
+        self._check(load().fmm_kernel_p2p(self._h, len(xi), _ptr(xi), _ptr(trg), len(xj), _ptr(xj), _ptr(qj),
+                                          _ptr(xp) if xp is not None else None), "fmm_kernel_p2p")
+        return trg
+
+    def kernel_p2m(self, Xc, x, q, M=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        M = self._cx(M, self.nterm)
+        self._check(load().fmm_kernel_p2m(self._h, _ptr(self._c3(Xc)), len(x), _ptr(x), _ptr(q),
+                                          _ptr(M.view(np.float64))), "fmm_kernel_p2m")
+        return M
+
+    def kernel_m2m(self, Xp, Xc, Mc, Mp=None):
+        Xc = np.ascontiguousarray(Xc, dtype=np.float64).reshape(-1, 3)
+        Mc = np.ascontiguousarray(Mc, dtype=np.complex128).reshape(len(Xc), self.nterm)
+        Mp = self._cx(Mp, self.nterm)
+        self._check(load().fmm_kernel_m2m(self._h, _ptr(self._c3(Xp)), len(Xc), _ptr(Xc),
+                                          _ptr(Mc.view(np.float64)), _ptr(Mp.view(np.float64))), "fmm_kernel_m2m")
+        return Mp
+
+    def kernel_m2l(self, Xi, Xj, Mj, Li=None, xper=None):
+        """Batched: Xi, Xj (k, 3), Mj (k, NTERM) -> Li (k, NTERM)."""
+        Xi = np.ascontiguousarray(Xi, dtype=np.float64).reshape(-1, 3)
+        Xj = np.ascontiguousarray(Xj, dtype=np.float64).reshape(-1, 3)
+        k = len(Xi)
+        Mj = np.ascontiguousarray(Mj, dtype=np.complex128).reshape(k, self.nterm)
+        Li = self._cx(Li, (k, self.nterm))
+        xp = None if xper is None else self._c3(xper)
+        self._check(load().fmm_kernel_m2l(self._h, k, _ptr(Xi), _ptr(Xj), _ptr(xp) if xp is not None else None,
+                                          _ptr(Mj.view(np.float64)), _ptr(Li.view(np.float64))), "fmm_kernel_m2l")
+        return Li
+
+    def kernel_l2l(self, Xc, Xp, Lp, Lc=None):
+        Lp = np.ascontiguousarray(Lp, dtype=np.complex128).reshape(self.nterm)
+        Lc = self._cx(Lc, self.nterm)
+        self._check(load().fmm_kernel_l2l(self._h, _ptr(self._c3(Xc)), _ptr(self._c3(Xp)),
+                                          _ptr(Lp.view(np.float64)), _ptr(Lc.view(np.float64))), "fmm_kernel_l2l")
+        return Lc
+
+    def kernel_l2p(self, Xc, L, x, trg=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        L = np.ascontiguousarray(L, dtype=np.complex128).reshape(self.nterm)
+        trg = np.zeros((len(x), 4)) if trg is None else np.array(trg, dtype=np.float64, order="C")
+        self._check(load().fmm_kernel_l2p(self._h, _ptr(self._c3(Xc)), _ptr(L.view(np.float64)), len(x), _ptr(x),
+                                          _ptr(trg)), "fmm_kernel_l2p")
+        return trg
+
+
 class Plan:
     """Host-only tree + interaction lists (fmm_plan_*): no device needed."""
 
@@ -399,6 +473,35 @@ class Plan:
                 self._h = ctypes.c_void_p()
         except Exception:
             pass
+
+    @classmethod
+    def tree_in_bounds(cls, x, ncrit, xmin, xmax):
+        """BuildTree::buildTree(bodies, buffer, bounds): the octree of x in the
+        given bounds, no lists (fmm_plan_build_tree)."""
+        L = load()
+        self = cls.__new__(cls)
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        self._h = ctypes.c_void_p()
+        lo = np.ascontiguousarray(xmin, dtype=np.float64)
+        hi = np.ascontiguousarray(xmax, dtype=np.float64)
+        rc = L.fmm_plan_build_tree(ctypes.byref(self._h), len(self.x), _ptr(self.x), ncrit, _ptr(lo), _ptr(hi))
+        if rc != 0:
+            raise FMMError("fmm_plan_build_tree failed (rc=%d)" % rc)
+        return self
+
+    @classmethod
+    def pair(cls, targets, sources, theta=0.4, nspawn=5000):
+        """Traversal::traverse(icells, jcells): the forest of a target plan and
+        a source plan with the lists of the pair traversal (fmm_plan_pair);
+        bodies are the targets' x then the sources' x."""
+        L = load()
+        self = cls.__new__(cls)
+        self.x = np.concatenate([targets.x, sources.x])
+        self._h = ctypes.c_void_p()
+        rc = L.fmm_plan_pair(ctypes.byref(self._h), targets._h, sources._h, theta, nspawn)
+        if rc != 0:
+            raise FMMError("fmm_plan_pair failed (rc=%d)" % rc)
+        return self
 
     @classmethod
     def merge(cls, plans):

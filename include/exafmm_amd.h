@@ -114,6 +114,22 @@ void fmm_plan_destroy(fmm_plan *pl);
 int fmm_plan_list_counts(const fmm_plan *pl, uint64_t *icell, int *nump2p, int *numm2l);
 int fmm_get_list_counts(fmm_ctx *c, uint64_t *icell, int *nump2p, int *numm2l);
 
+/* BuildTree::buildTree(bodies, buffer, bounds) (include/build_tree.h:186-209)
+ * in the caller's bounds (BoundBox::getBounds, include/bound_box.h:11-37):
+ * the octree only; its lists are empty until fmm_plan_pair, or a context's
+ * fmm_set_tree, builds them. */
+int fmm_plan_build_tree(fmm_plan **out, int n, const double *x, int ncrit, const double *xmin,
+                        const double *xmax);
+/* Traversal::traverse(icells, jcells, ...) with separate target and source
+ * trees (include/traversal.h:326-333; examples/fmm.cxx:69-72, the -j path):
+ * one forest of the two trees (target cells and bodies first) whose lists are
+ * the dual traversal from the pair of roots: the targets' M2L and P2P lists
+ * name source cells only.  A context given this plan (fmm_set_bodies_plan
+ * with x, q = the targets' then the sources' bodies, each in its plan's input
+ * order) evaluates the field of the sources at the targets; the source
+ * bodies' own targets stay zero.  Both plans free-space (images = 0). */
+int fmm_plan_pair(fmm_plan **out, const fmm_plan *targets, const fmm_plan *sources, double theta, int nspawn);
+
 /* fills n, ncell, nleaf, depth, pair counts, bounds and host times */
 int fmm_plan_stats(const fmm_plan *pl, fmm_stats *s);
 int fmm_plan_tree(const fmm_plan *pl, int *perm, int *iparent, int *ichild, int *nchild,
@@ -224,6 +240,33 @@ int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, do
  * (nt x 3); trg (nt x 4) is overwritten.  R^2 = 0 pairs are skipped like in
  * Kernel::P2P.  Uses IEEE sqrt and division, independently of the P2P kernel. */
 int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg);
+
+/* ---- Kernel operators (kernels/laplace.h:150-388) ----
+ * The reference's per-cell Kernel methods on cells the caller holds, each one
+ * device launch on the context's stream (synchronous); the context needs no
+ * bodies.  Expansions are [NTERM] complex values as interleaved doubles in
+ * the reference's normalisation; results are added (+=), as the reference's
+ * methods add into Ci->M, Ci->L and Body::TRG.  FP64 whatever the context's
+ * precision. */
+/* Kernel::P2P(Ci, Cj) (laplace.h:150-235): trgi[4i..4i+3] += potential and
+ * gradient at xi[i] of the nj sources xj, qj shifted by xper (the Kernel's
+ * Xperiodic; NULL = 0) with the context's eps2; R^2 = 0 pairs skipped. */
+int fmm_kernel_p2p(fmm_ctx *c, int ni, const double *xi, double *trgi, int nj, const double *xj,
+                   const double *qj, const double *xper);
+/* Kernel::P2M(C) (laplace.h:237-252): M += the multipole about Xc of n bodies. */
+int fmm_kernel_p2m(fmm_ctx *c, const double *Xc, int n, const double *x, const double *q, double *M);
+/* Kernel::M2M(Ci, C0) (laplace.h:254-290): Mp += the multipoles Mc (nchild x
+ * NTERM) of the children at Xc (nchild x 3) shifted to the parent at Xp. */
+int fmm_kernel_m2m(fmm_ctx *c, const double *Xp, int nchild, const double *Xc, const double *Mc, double *Mp);
+/* Kernel::M2L(Ci, Cj) (laplace.h:292-322), npair at once: Li[k] += the local
+ * expansion about Xi[k] of the multipole Mj[k] about Xj[k] + xper (NULL = 0). */
+int fmm_kernel_m2l(fmm_ctx *c, int npair, const double *Xi, const double *Xj, const double *xper,
+                   const double *Mj, double *Li);
+/* Kernel::L2L(Ci, C0) (laplace.h:324-357): Lc += the parent's Lp about Xp shifted to Xc. */
+int fmm_kernel_l2l(fmm_ctx *c, const double *Xc, const double *Xp, const double *Lp, double *Lc);
+/* Kernel::L2P(Ci) (laplace.h:359-388): trg (n x 4) += the local expansion L
+ * about Xc evaluated at the n points x. */
+int fmm_kernel_l2p(fmm_ctx *c, const double *Xc, const double *L, int n, const double *x, double *trg);
 
 /* Targets (n x 4). input_order != 0 returns them in the order of the x passed
  * to fmm_set_bodies, else in tree order (exafmm-beta leaves bodies sorted). */

@@ -11,11 +11,13 @@
 //   UpDownPass::downwardPass   include/up_down_pass.h:63-75  (L2L + L2P; TRG += results)
 //   Traversal::direct          include/traversal.h:355-377
 //   Dataset, Verify, Args      include/dataset.h, verify.h, args.h
+//   Kernel::P2P/P2M/M2M/M2L/L2L/L2P  kernels/laplace.h:150-388 (per-cell calls, one launch each)
 // Body and Cell keep the reference's fields; Cell::M and Cell::L are filled
 // after the passes (reference normalisation).  Errors that the reference
 // would hit as undefined behaviour or abort() are reported by throwing
-// exafmm_amd::Error.  Per-pair Kernel::P2P/M2L/... calls are not provided:
-// the passes replace them (one device launch per phase).
+// exafmm_amd::Error.  The passes run one device launch per phase; the
+// per-cell Kernel methods are for callers that drive their own traversal
+// (each is one launch, so a whole solve should use the passes).
 #pragma once
 #include <getopt.h>
 #include <sys/time.h>
@@ -138,6 +140,10 @@ class BoundBox {
       return b;
     }
     b.Xmin = b.Xmax = bodies.front().X;
+    return getBounds(bodies, b);
+  }
+  //! include/bound_box.h:28-37: widen given bounds by the bodies (same drift)
+  Bounds getBounds(Bodies &bodies, Bounds b) {
     for (B_iter B = bodies.begin(); B != bodies.end(); B++)
       for (int d = 0; d < 3; d++) {
         const real_t lo = b.Xmin[d] - 1e-5, hi = b.Xmax[d] + 1e-5;
@@ -156,15 +162,16 @@ class BuildTree {
 
  public:
   explicit BuildTree(int _ncrit) : ncrit(_ncrit) {}
-  Cells buildTree(Bodies &bodies, Bodies &buffer, Bounds) {
+  //! the octree of bodies in the given bounds (the reference's box from them)
+  Cells buildTree(Bodies &bodies, Bodies &buffer, Bounds bounds) {
     logger::startTimer("Grow tree");
     const int n = (int)bodies.size();
     std::vector<double> x(3 * (size_t)n);
     for (int i = 0; i < n; i++)
       for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = bodies[i].X[d];
     fmm_plan *pl = nullptr;
-    // theta/nspawn only shape the plan's lists, which are not used here
-    if (fmm_plan_build(&pl, n, x.data(), ncrit, 0.4, 5000) != FMM_OK) throw Error("buildTree failed");
+    if (fmm_plan_build_tree(&pl, n, x.data(), ncrit, bounds.Xmin.d, bounds.Xmax.d) != FMM_OK)
+      throw Error("buildTree failed");
     fmm_stats s;
     fmm_plan_stats(pl, &s);
     const int nc = s.ncell;
@@ -217,16 +224,21 @@ class Kernel {
   complex_t wavek;
   vec3 Xperiodic;
   fmm_ctx *ctx = nullptr;
+  fmm_ctx *dctx = nullptr;    // Traversal::direct's own context (the solve's tree stays up)
   real_t theta = 0.4;
   int nspawn = 5000;
   int images = 0;
   vec3 cycle = 2 * M_PI;      // the period the context was made with (images > 0)
   Cells *uploaded = nullptr;  // cells whose tree is on the device
+  // traverse(icells, jcells): the device holds the forest of both trees,
+  // target cells and bodies first (-1: one tree)
+  int pair_cells = -1, pair_bodies = -1;
 
   Kernel(int _P, real_t _eps2, complex_t _wavek, int device = 0)
       : P(_P), NTERM(_P * (_P + 1) / 2), eps2(_eps2), wavek(_wavek), Xperiodic(0.0), dev(device) {}
   ~Kernel() {
     if (ctx) fmm_destroy(ctx);
+    if (dctx) fmm_destroy(dctx);
   }
   Kernel(const Kernel &) = delete;
   Kernel &operator=(const Kernel &) = delete;
@@ -256,14 +268,98 @@ class Kernel {
       throw Error(m);
     }
   }
-  //! drop the device context (remade by the next pass)
+  //! drop the device contexts (remade by the next pass)
   void release() {
     if (ctx) fmm_destroy(ctx);
-    ctx = nullptr;
+    if (dctx) fmm_destroy(dctx);
+    ctx = dctx = nullptr;
     uploaded = nullptr;
+    pair_cells = pair_bodies = -1;
   }
-  void check(int rc, const char *what) const {
-    if (rc != FMM_OK) throw Error(std::string(what) + ": " + fmm_last_error(ctx));
+  void check(int rc, const char *what, fmm_ctx *on = nullptr) const {
+    if (rc != FMM_OK) throw Error(std::string(what) + ": " + fmm_last_error(on ? on : ctx));
+  }
+  fmm_ctx *context() {
+    if (!ctx) configure(theta, nspawn, images, cycle);
+    return ctx;
+  }
+
+  // ---- kernels/laplace.h:150-388: the per-cell operators on the caller's cells
+  //! laplace.h:150-235: TRG of Ci's bodies += field of Cj's bodies shifted by Xperiodic
+  void P2P(C_iter Ci, C_iter Cj) {
+    const int ni = Ci->NBODY, nj = Cj->NBODY;
+    if (ni <= 0 || nj <= 0) return;
+    std::vector<double> xi(3 * (size_t)ni), xj(3 * (size_t)nj), qj(nj), trg(4 * (size_t)ni, 0.0);
+    for (int i = 0; i < ni; i++)
+      for (int d = 0; d < 3; d++) xi[3 * (size_t)i + d] = Ci->BODY[i].X[d];
+    for (int j = 0; j < nj; j++) {
+      for (int d = 0; d < 3; d++) xj[3 * (size_t)j + d] = Cj->BODY[j].X[d];
+      qj[j] = Cj->BODY[j].SRC;
+    }
+    check(fmm_kernel_p2p(context(), ni, xi.data(), trg.data(), nj, xj.data(), qj.data(), Xperiodic.d),
+          "Kernel::P2P");
+    for (int i = 0; i < ni; i++)
+      for (int k = 0; k < 4; k++) Ci->BODY[i].TRG[k] += trg[4 * (size_t)i + k];
+  }
+  //! laplace.h:237-252: C->M += multipole of C's bodies about C->X
+  void P2M(C_iter C) {
+    sized(C->M);
+    const int n = C->NBODY;
+    std::vector<double> x(3 * (size_t)n), q(n);
+    for (int i = 0; i < n; i++) {
+      for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = C->BODY[i].X[d];
+      q[i] = C->BODY[i].SRC;
+    }
+    check(fmm_kernel_p2m(context(), C->X.d, n, x.data(), q.data(), reinterpret_cast<double *>(C->M.data())),
+          "Kernel::P2M");
+  }
+  //! laplace.h:254-290: Ci->M += the multipoles of Ci's children (C0 + ICHILD ...) shifted to Ci
+  void M2M(C_iter Ci, C_iter C0) {
+    sized(Ci->M);
+    const int nc = Ci->NCHILD;
+    std::vector<double> X(3 * (size_t)nc);
+    std::vector<complex_t> Mc((size_t)nc * NTERM);
+    for (int k = 0; k < nc; k++) {
+      C_iter Cj = C0 + Ci->ICHILD + k;
+      for (int d = 0; d < 3; d++) X[3 * (size_t)k + d] = Cj->X[d];
+      for (int u = 0; u < NTERM && u < (int)Cj->M.size(); u++) Mc[(size_t)k * NTERM + u] = Cj->M[u];
+    }
+    check(fmm_kernel_m2m(context(), Ci->X.d, nc, X.data(), reinterpret_cast<const double *>(Mc.data()),
+                         reinterpret_cast<double *>(Ci->M.data())),
+          "Kernel::M2M");
+  }
+  //! laplace.h:292-322: Ci->L += local expansion of Cj->M (source shifted by Xperiodic)
+  void M2L(C_iter Ci, C_iter Cj) {
+    sized(Ci->L);
+    std::vector<complex_t> Mj(NTERM);
+    for (int u = 0; u < NTERM && u < (int)Cj->M.size(); u++) Mj[u] = Cj->M[u];
+    check(fmm_kernel_m2l(context(), 1, Ci->X.d, Cj->X.d, Xperiodic.d, reinterpret_cast<const double *>(Mj.data()),
+                         reinterpret_cast<double *>(Ci->L.data())),
+          "Kernel::M2L");
+  }
+  //! laplace.h:324-357: Ci->L += the local expansion of its parent (C0 + IPARENT) shifted to Ci
+  void L2L(C_iter Ci, C_iter C0) {
+    sized(Ci->L);
+    C_iter Cp = C0 + Ci->IPARENT;
+    std::vector<complex_t> Lp(NTERM);
+    for (int u = 0; u < NTERM && u < (int)Cp->L.size(); u++) Lp[u] = Cp->L[u];
+    check(fmm_kernel_l2l(context(), Ci->X.d, Cp->X.d, reinterpret_cast<const double *>(Lp.data()),
+                         reinterpret_cast<double *>(Ci->L.data())),
+          "Kernel::L2L");
+  }
+  //! laplace.h:359-388: TRG of Ci's bodies += Ci->L evaluated at them
+  void L2P(C_iter Ci) {
+    const int n = Ci->NBODY;
+    if (n <= 0) return;
+    std::vector<complex_t> L(NTERM);
+    for (int u = 0; u < NTERM && u < (int)Ci->L.size(); u++) L[u] = Ci->L[u];
+    std::vector<double> x(3 * (size_t)n), trg(4 * (size_t)n, 0.0);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = Ci->BODY[i].X[d];
+    check(fmm_kernel_l2p(context(), Ci->X.d, reinterpret_cast<const double *>(L.data()), n, x.data(), trg.data()),
+          "Kernel::L2P");
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < 4; k++) Ci->BODY[i].TRG[k] += trg[4 * (size_t)i + k];
   }
   //! upload the tree held by cells (bodies in tree order, as buildTree leaves them)
   void upload(Cells &cells) {
@@ -290,10 +386,36 @@ class Kernel {
                        nb.data(), X.data(), R.data()),
           "fmm_set_tree");
     uploaded = &cells;
+    pair_cells = pair_bodies = -1;
+  }
+  //! the context for direct sums, made with the same parameters
+  fmm_ctx *direct_context() {
+    if (!dctx) {
+      fmm_params p;
+      fmm_default_params(&p);
+      p.P = P;
+      p.eps2 = eps2;
+      p.theta = theta;
+      p.nspawn = nspawn;
+      p.images = images;
+      for (int d = 0; d < 3; d++) p.cycle[d] = cycle[d];
+      p.device = dev;
+      const int rc = fmm_create(&dctx, &p);
+      if (rc != FMM_OK) {
+        const std::string m = dctx ? fmm_last_error(dctx) : "fmm_create failed";
+        if (dctx) fmm_destroy(dctx);
+        dctx = nullptr;
+        throw Error(m);
+      }
+    }
+    return dctx;
   }
 
  private:
   int dev;
+  void sized(std::vector<complex_t> &E) const {
+    if ((int)E.size() < NTERM) E.resize(NTERM, 0.0);
+  }
 };
 
 //! include/up_down_pass.h:46-99
@@ -322,13 +444,17 @@ class UpDownPass {
     if (!cells.empty()) {
       if (kernel.uploaded != &cells) throw Error("downwardPass: cells differ from the upward pass");
       kernel.check(fmm_downward(kernel.ctx), "fmm_downward");
+      // the device may hold a forest (traverse(icells, jcells)): these cells
+      // and their bodies come first
+      fmm_stats st;
+      fmm_get_stats(kernel.ctx, &st);
       const int n = cells[0].NBODY;
-      std::vector<double> trg(4 * (size_t)n);
+      std::vector<double> trg(4 * (size_t)st.n);
       kernel.check(fmm_get_targets(kernel.ctx, trg.data(), 0), "fmm_get_targets");
       B_iter B0 = cells[0].BODY;
       for (int i = 0; i < n; i++)
         for (int k = 0; k < 4; k++) B0[i].TRG[k] += trg[4 * (size_t)i + k];
-      std::vector<complex_t> L(cells.size() * (size_t)kernel.NTERM);
+      std::vector<complex_t> L((size_t)st.ncell * kernel.NTERM);
       kernel.check(fmm_get_expansions(kernel.ctx, nullptr, reinterpret_cast<double *>(L.data())),
                    "fmm_get_expansions");
       for (size_t c = 0; c < cells.size(); c++)
@@ -354,7 +480,7 @@ class UpDownPass {
   }
 };
 
-//! include/traversal.h:291-377 (icells == jcells)
+//! include/traversal.h:291-377
 class Traversal {
   Kernel &kernel;
   const real_t theta;
@@ -391,10 +517,13 @@ class Traversal {
   void initWeight(Cells &) {}
   void traverse(Cells &icells, Cells &jcells, vec3 cycle, bool, real_t = 1) {
     if (icells.empty() || jcells.empty()) return;
-    if (&icells != &jcells) throw Error("traverse: separate target and source trees are not supported");
+    if (&icells != &jcells) {
+      traverse_pair(icells, jcells);
+      return;
+    }
     logger::startTimer("Traverse");
     use_cycle(cycle);
-    if (kernel.uploaded != &icells) kernel.upload(icells);
+    if (kernel.uploaded != &icells || kernel.pair_cells >= 0) kernel.upload(icells);
     kernel.check(fmm_p2p(kernel.ctx), "fmm_p2p");
     kernel.check(fmm_m2l(kernel.ctx), "fmm_m2l");
     kernel.check(fmm_synchronize(kernel.ctx), "fmm_synchronize");
@@ -407,8 +536,79 @@ class Traversal {
     }
     logger::stopTimer("Traverse");
   }
+  //! Traversal::traverse(icells, jcells) with two trees (traversal.h:326-333,
+  //! examples/fmm.cxx:69-72): the device gets the forest of both trees and the
+  //! pair traversal's lists; M of both (the upward pass again, on the device),
+  //! then P2P and M2L into icells.  downwardPass(icells) finishes the solve.
+  void traverse_pair(Cells &icells, Cells &jcells) {
+    if (images != 0) throw Error("traverse: separate source trees need images = 0");
+    logger::startTimer("Traverse");
+    fmm_plan *pi = tree_plan(icells), *pj = tree_plan(jcells), *pp = nullptr;
+    const int rc = fmm_plan_pair(&pp, pi, pj, theta, nspawn);
+    fmm_plan_destroy(pi);
+    fmm_plan_destroy(pj);
+    if (rc != FMM_OK) throw Error("traverse: fmm_plan_pair failed");
+    const int ni = icells[0].NBODY, nj = jcells[0].NBODY;
+    std::vector<double> x(3 * ((size_t)ni + nj)), q((size_t)ni + nj);
+    auto put = [&](Cells &cells, int n, size_t off) {
+      B_iter B0 = cells[0].BODY;
+      for (int i = 0; i < n; i++) {
+        for (int d = 0; d < 3; d++) x[3 * (off + i) + d] = B0[i].X[d];
+        q[off + i] = B0[i].SRC;
+      }
+    };
+    put(icells, ni, 0);
+    put(jcells, nj, ni);
+    fmm_ctx *c = kernel.context();
+    const int r1 = fmm_set_bodies_plan(c, pp, x.data(), q.data());
+    fmm_plan_destroy(pp);
+    kernel.check(r1, "fmm_set_bodies_plan");
+    kernel.uploaded = &icells;
+    kernel.pair_cells = (int)icells.size();
+    kernel.pair_bodies = ni;
+    kernel.check(fmm_upward(c), "fmm_upward");
+    kernel.check(fmm_p2p(c), "fmm_p2p");
+    kernel.check(fmm_m2l(c), "fmm_m2l");
+    kernel.check(fmm_synchronize(c), "fmm_synchronize");
+    fmm_stats st;
+    fmm_get_stats(c, &st);
+    std::vector<int> np2p(st.ncell), nm2l(st.ncell);
+    kernel.check(fmm_get_list_counts(c, nullptr, np2p.data(), nm2l.data()), "fmm_get_list_counts");
+    for (size_t k = 0; k < icells.size(); k++) {
+      icells[k].numP2P += np2p[k];
+      icells[k].numM2L += nm2l[k];
+    }
+    logger::stopTimer("Traverse");
+  }
+  //! a caller's tree (bodies in tree order) as a host plan without lists
+  static fmm_plan *tree_plan(Cells &cells) {
+    const int nc = (int)cells.size(), n = cells[0].NBODY;
+    B_iter B0 = cells[0].BODY;
+    std::vector<double> x(3 * (size_t)n), X(3 * (size_t)nc), R(nc);
+    std::vector<int> ip(nc), ic(nc), nch(nc), ib(nc), nb(nc);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) x[3 * (size_t)i + d] = B0[i].X[d];
+    for (int k = 0; k < nc; k++) {
+      ip[k] = cells[k].IPARENT;
+      ic[k] = cells[k].ICHILD;
+      nch[k] = cells[k].NCHILD;
+      ib[k] = cells[k].IBODY;
+      nb[k] = cells[k].NBODY;
+      for (int d = 0; d < 3; d++) X[3 * (size_t)k + d] = cells[k].X[d];
+      R[k] = cells[k].R;
+    }
+    fmm_plan *pl = nullptr;
+    char why[256] = {0};
+    // the lists of this plan are not used (the pair traversal replaces them):
+    // a theta that accepts every separated pair keeps them short
+    if (fmm_plan_from_tree(&pl, n, x.data(), nc, ip.data(), ic.data(), nch.data(), ib.data(), nb.data(), X.data(),
+                           R.data(), 1e100, 1 << 30, why, (int)sizeof why) != FMM_OK)
+      throw Error(std::string("traverse: tree rejected: ") + why);
+    return pl;
+  }
   //! direct sum of jbodies (and, with images > 0, of their copies shifted by
-  //! (ix, iy, iz) * cycle) at the positions of ibodies (TRG += ), on the device
+  //! (ix, iy, iz) * cycle) at the positions of ibodies (TRG += ), on the
+  //! device, in a context of its own (the solve's tree stays uploaded)
   void direct(Bodies &ibodies, Bodies &jbodies, vec3 cycle) {
     const int nj = (int)jbodies.size(), ni = (int)ibodies.size();
     if (!kernel.ctx) kernel.configure(theta, nspawn, images, images ? cycle : kernel.cycle);
@@ -420,14 +620,13 @@ class Traversal {
     }
     for (int i = 0; i < ni; i++)
       for (int d = 0; d < 3; d++) xi[3 * (size_t)i + d] = ibodies[i].X[d];
-    // the context's bodies become jbodies (a one-leaf tree): only the direct kernel runs
+    // the direct context's bodies are jbodies (a one-leaf tree): only the direct kernel runs
+    fmm_ctx *c = kernel.direct_context();
     const int ip = 0, ic = 0, nch = 0, ib = 0;
     double X[3] = {0, 0, 0}, R = 1.0;
-    kernel.check(fmm_set_tree(kernel.ctx, nj, xj.data(), qj.data(), nj ? 1 : 0, &ip, &ic, &nch, &ib, &nj,
-                              X, &R),
-                 "fmm_set_tree");
-    kernel.uploaded = nullptr;
-    kernel.check(fmm_direct(kernel.ctx, ni, xi.data(), trg.data()), "fmm_direct");
+    kernel.check(fmm_set_tree(c, nj, xj.data(), qj.data(), nj ? 1 : 0, &ip, &ic, &nch, &ib, &nj, X, &R),
+                 "fmm_set_tree", c);
+    kernel.check(fmm_direct(c, ni, xi.data(), trg.data()), "fmm_direct", c);
     for (int i = 0; i < ni; i++)
       for (int k = 0; k < 4; k++) ibodies[i].TRG[k] += trg[4 * (size_t)i + k];
   }

@@ -372,6 +372,17 @@ void orc_bounds(int n, const double *x, double *xmin, double *xmax) {
   }
 }
 
+/* bound_box.h:28-37: getBounds(bodies, bounds) widens given bounds */
+void orc_bounds_update(int n, const double *x, double *xmin, double *xmax) {
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      double a = xmin[d] - 1e-5, b = xmax[d] + 1e-5;
+      xmin[d] = x[3 * i + d] < a ? x[3 * i + d] : a;
+      xmax[d] = x[3 * i + d] > b ? x[3 * i + d] : b;
+    }
+  }
+}
+
 /* ---------------- tree (build_tree.h) ---------------- */
 typedef struct onode {
   int ibody, nbody, nnode;
@@ -559,29 +570,32 @@ struct orc_lists {
   unsigned char *type;
 };
 
+/* t: the target tree (Ci), tj: the source tree (Cj); the same tree for
+ * traverse(cells, cells), two trees for traverse(icells, jcells) */
 typedef struct {
-  const orc_tree *t;
+  const orc_tree *t, *tj;
   double theta;
   int nspawn;
   double xper[3];
   orc_lists *out;
   /* when eval is set the pairs are evaluated immediately instead */
   const orc_kernel *k;
-  const double *x, *q;
+  const double *x, *q;    /* target bodies (and, one tree, the sources) */
+  const double *xj, *qj;  /* source bodies */
   double *trg;
-  cplx *M, *L;
+  cplx *M, *L;            /* M of the source tree, L of the target tree */
 } trav_ctx;
 
 static void emit(trav_ctx *c, int ci, int cj, int type) {
   if (c->k) {
-    const orc_tree *t = c->t;
+    const orc_tree *t = c->t, *tj = c->tj;
     if (type == 0) {
-      orc_m2l(c->k, t->X + 3 * ci, t->X + 3 * cj, c->xper, (const double *)(c->M + (size_t)cj * c->k->NTERM),
+      orc_m2l(c->k, t->X + 3 * ci, tj->X + 3 * cj, c->xper, (const double *)(c->M + (size_t)cj * c->k->NTERM),
               (double *)(c->L + (size_t)ci * c->k->NTERM));
     } else {
-      int bi = t->ibody[ci], bj = t->ibody[cj];
-      orc_p2p(c->k, t->nbody[ci], c->x + 3 * (size_t)bi, c->trg + 4 * (size_t)bi, t->nbody[cj],
-              c->x + 3 * (size_t)bj, c->q + bj, c->xper);
+      int bi = t->ibody[ci], bj = tj->ibody[cj];
+      orc_p2p(c->k, t->nbody[ci], c->x + 3 * (size_t)bi, c->trg + 4 * (size_t)bi, tj->nbody[cj],
+              c->xj + 3 * (size_t)bj, c->qj + bj, c->xper);
     }
   }
   orc_lists *l = c->out;
@@ -601,10 +615,11 @@ static void emit(trav_ctx *c, int ci, int cj, int type) {
 
 static void dual_tree(trav_ctx *c, int Ci, int Cj);
 
-/* TraverseRange::operator(), run serially in task order: traversal.h:196-229 */
+/* TraverseRange::operator(), run serially in task order: traversal.h:196-229
+ * (ranges of two different trees never begin at the same cell) */
 static void traverse_range(trav_ctx *c, int ib, int ie, int jb, int je) {
   if (ie - ib == 1 || je - jb == 1) {
-    if (ib == jb) {
+    if (ib == jb && c->t == c->tj) {
       dual_tree(c, ib, jb);
     } else {
       for (int i = ib; i < ie; i++)
@@ -621,32 +636,32 @@ static void traverse_range(trav_ctx *c, int ib, int ie, int jb, int je) {
 
 /* splitCell: traversal.h:90-114 */
 static void split_cell(trav_ctx *c, int Ci, int Cj) {
-  const orc_tree *t = c->t;
-  if (t->nchild[Cj] == 0) {
+  const orc_tree *t = c->t, *tj = c->tj;
+  if (tj->nchild[Cj] == 0) {
     for (int ci = t->ichild[Ci]; ci < t->ichild[Ci] + t->nchild[Ci]; ci++) dual_tree(c, ci, Cj);
   } else if (t->nchild[Ci] == 0) {
-    for (int cj = t->ichild[Cj]; cj < t->ichild[Cj] + t->nchild[Cj]; cj++) dual_tree(c, Ci, cj);
-  } else if (t->nbody[Ci] + t->nbody[Cj] >= c->nspawn || Ci == Cj) {
-    traverse_range(c, t->ichild[Ci], t->ichild[Ci] + t->nchild[Ci], t->ichild[Cj],
-                   t->ichild[Cj] + t->nchild[Cj]);
-  } else if (t->R[Ci] >= t->R[Cj]) {
+    for (int cj = tj->ichild[Cj]; cj < tj->ichild[Cj] + tj->nchild[Cj]; cj++) dual_tree(c, Ci, cj);
+  } else if (t->nbody[Ci] + tj->nbody[Cj] >= c->nspawn || (t == tj && Ci == Cj)) {
+    traverse_range(c, t->ichild[Ci], t->ichild[Ci] + t->nchild[Ci], tj->ichild[Cj],
+                   tj->ichild[Cj] + tj->nchild[Cj]);
+  } else if (t->R[Ci] >= tj->R[Cj]) {
     for (int ci = t->ichild[Ci]; ci < t->ichild[Ci] + t->nchild[Ci]; ci++) dual_tree(c, ci, Cj);
   } else {
-    for (int cj = t->ichild[Cj]; cj < t->ichild[Cj] + t->nchild[Cj]; cj++) dual_tree(c, Ci, cj);
+    for (int cj = tj->ichild[Cj]; cj < tj->ichild[Cj] + tj->nchild[Cj]; cj++) dual_tree(c, Ci, cj);
   }
 }
 
 /* dualTreeTraversal: traversal.h:117-177 (EXAFMM_NO_P2P off) */
 static void dual_tree(trav_ctx *c, int Ci, int Cj) {
-  const orc_tree *t = c->t;
+  const orc_tree *t = c->t, *tj = c->tj;
   double dX[3];
-  for (int d = 0; d < 3; d++) dX[d] = t->X[3 * Ci + d] - t->X[3 * Cj + d] - c->xper[d];
+  for (int d = 0; d < 3; d++) dX[d] = t->X[3 * Ci + d] - tj->X[3 * Cj + d] - c->xper[d];
   double RT2 = norm3(dX) * c->theta * c->theta;
-  double Rs = t->R[Ci] + t->R[Cj];
+  double Rs = t->R[Ci] + tj->R[Cj];
   if (RT2 > Rs * Rs * (1 - 1e-3)) {
     emit(c, Ci, Cj, 0);
-  } else if (t->nchild[Ci] == 0 && t->nchild[Cj] == 0) {
-    if (t->nbody[Cj] == 0)
+  } else if (t->nchild[Ci] == 0 && tj->nchild[Cj] == 0) {
+    if (tj->nbody[Cj] == 0)
       emit(c, Ci, Cj, 0);
     else
       emit(c, Ci, Cj, 1);
@@ -660,7 +675,26 @@ orc_lists *orc_traverse(const orc_tree *t, double theta, int nspawn, const doubl
   if (t->ncell == 0) return l;
   trav_ctx c;
   memset(&c, 0, sizeof c);
-  c.t = t;
+  c.t = c.tj = t;
+  c.theta = theta;
+  c.nspawn = nspawn;
+  if (xper) memcpy(c.xper, xper, sizeof c.xper);
+  c.out = l;
+  dual_tree(&c, 0, 0);
+  return l;
+}
+
+/* Traversal::traverse(icells, jcells) with icells != jcells
+ * (include/traversal.h:326-333, examples/fmm.cxx:69-72): the dual traversal
+ * from the pair of roots of two trees; ci indexes ti, cj indexes tj */
+orc_lists *orc_traverse_pair(const orc_tree *ti, const orc_tree *tj, double theta, int nspawn,
+                             const double *xper) {
+  orc_lists *l = (orc_lists *)calloc(1, sizeof(orc_lists));
+  if (ti->ncell == 0 || tj->ncell == 0) return l;
+  trav_ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = ti;
+  c.tj = tj;
   c.theta = theta;
   c.nspawn = nspawn;
   if (xper) memcpy(c.xper, xper, sizeof c.xper);
@@ -757,12 +791,12 @@ void orc_fmm(const orc_kernel *k, const orc_tree *t, const double *x, const doub
   memset(L, 0, sz);
   trav_ctx c;
   memset(&c, 0, sizeof c);
-  c.t = t;
+  c.t = c.tj = t;
   c.theta = theta;
   c.nspawn = nspawn;
   c.k = k;
-  c.x = x;
-  c.q = q;
+  c.x = c.xj = x;
+  c.q = c.qj = q;
   c.trg = trg;
   c.M = M;
   c.L = L;
@@ -770,6 +804,36 @@ void orc_fmm(const orc_kernel *k, const orc_tree *t, const double *x, const doub
   orc_downward(k, t, x, (double *)L, trg);
   if (!Mout) free(M);
   if (!Lout) free(L);
+}
+
+/* examples/fmm.cxx:59-77 with args.IneJ: upward pass of both trees, the
+ * pair traversal (M2L from the source tree's multipoles into the target
+ * tree's locals, P2P from source leaves to target leaves), downward pass of
+ * the target tree.  x, q: target bodies; xj, qj: source bodies (tree order). */
+void orc_fmm_pair(const orc_kernel *k, const orc_tree *ti, const double *x, const orc_tree *tj,
+                  const double *xj, const double *qj, double theta, int nspawn, double *trg) {
+  if (ti->ncell == 0 || tj->ncell == 0) return;
+  const size_t szi = sizeof(cplx) * (size_t)ti->ncell * k->NTERM;
+  const size_t szj = sizeof(cplx) * (size_t)tj->ncell * k->NTERM;
+  cplx *M = (cplx *)malloc(szj), *L = (cplx *)calloc(1, szi);
+  orc_upward(k, tj, xj, qj, (double *)M);
+  trav_ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = ti;
+  c.tj = tj;
+  c.theta = theta;
+  c.nspawn = nspawn;
+  c.k = k;
+  c.x = x;
+  c.xj = xj;
+  c.qj = qj;
+  c.trg = trg;
+  c.M = M;
+  c.L = L;
+  dual_tree(&c, 0, 0);
+  orc_downward(k, ti, x, (double *)L, trg);
+  free(M);
+  free(L);
 }
 
 /* Traversal::traversePeriodic  include/traversal.h:233-287 (target Ci0 = the
@@ -819,12 +883,12 @@ void orc_fmm_periodic(const orc_kernel *k, const orc_tree *t, const double *x, c
   memset(L, 0, sz);
   trav_ctx c;
   memset(&c, 0, sizeof c);
-  c.t = t;
+  c.t = c.tj = t;
   c.theta = theta;
   c.nspawn = nspawn;
   c.k = k;
-  c.x = x;
-  c.q = q;
+  c.x = c.xj = x;
+  c.q = c.qj = q;
   c.trg = trg;
   c.M = M;
   c.L = L;
@@ -851,10 +915,10 @@ void orc_eval_lists(const orc_kernel *k, const orc_tree *t, const double *x, con
                     double *L, const double *M) {
   trav_ctx c;
   memset(&c, 0, sizeof c);
-  c.t = t;
+  c.t = c.tj = t;
   c.k = k;
-  c.x = x;
-  c.q = q;
+  c.x = c.xj = x;
+  c.q = c.qj = q;
   c.trg = trg;
   c.M = (cplx *)M;
   c.L = (cplx *)L;

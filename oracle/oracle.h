@@ -50,6 +50,8 @@ void orc_l2p(const orc_kernel *k, const double *Xc, const double *L, int n,
 
 /* BoundBox::getBounds(Bodies&)  include/bound_box.h:11-25 (1e-5 drift kept) */
 void orc_bounds(int n, const double *x, double *xmin, double *xmax);
+/* getBounds(bodies, bounds) (bound_box.h:28-37): xmin/xmax widened in place */
+void orc_bounds_update(int n, const double *x, double *xmin, double *xmax);
 
 /* BuildTree::buildTree  include/build_tree.h:168-209.  x is in input order;
  * the tree records perm[i] = input index of the i-th body in tree order. */
@@ -67,6 +69,10 @@ void orc_tree_get(const orc_tree *t, int *perm, int *iparent, int *ichild,
  * type[k] = 0 for M2L, 1 for P2P. */
 orc_lists *orc_traverse(const orc_tree *t, double theta, int nspawn,
                         const double *xper);
+/* Traversal::traverse(icells, jcells), two trees (traversal.h:326-333):
+ * ci indexes ti (targets), cj indexes tj (sources) */
+orc_lists *orc_traverse_pair(const orc_tree *ti, const orc_tree *tj, double theta,
+                             int nspawn, const double *xper);
 void orc_lists_destroy(orc_lists *l);
 int64_t orc_lists_count(const orc_lists *l, int type);
 void orc_lists_get(const orc_lists *l, int *ci, int *cj, int *type);
@@ -79,6 +85,11 @@ void orc_lists_get(const orc_lists *l, int *ci, int *cj, int *type);
 void orc_fmm(const orc_kernel *k, const orc_tree *t, const double *x,
              const double *q, double theta, int nspawn, double *trg,
              double *M, double *L);
+/* examples/fmm.cxx -j: targets x (tree ti order) from sources xj, qj (tree tj
+ * order); trg accumulated for the targets */
+void orc_fmm_pair(const orc_kernel *k, const orc_tree *ti, const double *x,
+                  const orc_tree *tj, const double *xj, const double *qj,
+                  double theta, int nspawn, double *trg);
 
 /* orc_fmm with periodic images (images > 0): Traversal::traverse runs the
  * dual traversal for the 27 shifts (ix, iy, iz) * cycle, ix..iz in -1..1

@@ -64,6 +64,7 @@ def lib():
         L.orc_l2l.argtypes = [_vp, _dp, _dp, _dp, _dp]
         L.orc_l2p.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_bounds.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_bounds_update.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_tree_build.restype = _vp
         L.orc_tree_build.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
         L.orc_tree_destroy.argtypes = [_vp]
@@ -73,6 +74,9 @@ def lib():
                                    ctypes.POINTER(ctypes.c_uint64)]
         L.orc_traverse.restype = _vp
         L.orc_traverse.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_traverse_pair.restype = _vp
+        L.orc_traverse_pair.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_fmm_pair.argtypes = [_vp, _vp, _dp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
         L.orc_lists_destroy.argtypes = [_vp]
         L.orc_lists_count.restype = ctypes.c_int64
         L.orc_lists_count.argtypes = [_vp, ctypes.c_int]
@@ -201,11 +205,28 @@ class Kernel:
         return trg
 
 
-def bounds(x):
+def bounds(x, start=None):
+    """BoundBox::getBounds(bodies), or getBounds(bodies, start) widening the
+    given (lo, hi) (include/bound_box.h:11-37)."""
     x = _c(x)
-    lo, hi = np.zeros(3), np.zeros(3)
-    lib().orc_bounds(len(x), _ptr(x), _ptr(lo), _ptr(hi))
+    if start is None:
+        lo, hi = np.zeros(3), np.zeros(3)
+        lib().orc_bounds(len(x), _ptr(x), _ptr(lo), _ptr(hi))
+    else:
+        lo, hi = _c(start[0]).copy(), _c(start[1]).copy()
+        lib().orc_bounds_update(len(x), _ptr(x), _ptr(lo), _ptr(hi))
     return lo, hi
+
+
+def ine_j_bodies(n, seed_i=0, seed_j=1, dist="cube"):
+    """examples/fmm.cxx:37-47 with -j: targets of seed 0 with x -> (x + pi) / 2,
+    sources of seed 1 with x -> (x - pi) / 2 (for the cube: targets in
+    [0, pi] and sources in [-pi, 0] along x), charges of each set by its own seed."""
+    gen = data_cube if dist == "cube" else data_plummer
+    xi, xj = gen(n, seed_i), gen(n, seed_j)
+    xi[:, 0] = (xi[:, 0] + np.pi) * 0.5
+    xj[:, 0] = (xj[:, 0] - np.pi) * 0.5
+    return xi, data_source(n, seed_i), xj, data_source(n, seed_j)
 
 
 class Tree:
@@ -247,6 +268,26 @@ class Tree:
         L.orc_lists_get(h, _ptr(ci, _ip), _ptr(cj, _ip), _ptr(ty, _ip))
         L.orc_lists_destroy(h)
         return ci, cj, ty
+
+    def traverse_pair(self, sources, theta=0.4, nspawn=5000, xper=None):
+        """Traversal::traverse(icells, jcells): this tree's cells as targets,
+        `sources` (another Tree) as sources; ci indexes this tree, cj the other."""
+        L = lib()
+        xp = _c(xper if xper is not None else np.zeros(3))
+        h = L.orc_traverse_pair(self.h, sources.h, theta, nspawn, _ptr(xp))
+        n = L.orc_lists_count(h, -1)
+        ci, cj, ty = (np.zeros(n, np.int32) for _ in range(3))
+        L.orc_lists_get(h, _ptr(ci, _ip), _ptr(cj, _ip), _ptr(ty, _ip))
+        L.orc_lists_destroy(h)
+        return ci, cj, ty
+
+    def fmm_pair(self, kernel, x_sorted, sources, xj_sorted, qj_sorted, theta=0.4, nspawn=5000):
+        """orc_fmm_pair: the field of the source tree's bodies at this tree's bodies."""
+        xs, xj, qj = _c(x_sorted), _c(xj_sorted), _c(qj_sorted)
+        trg = np.zeros((self.n, 4))
+        lib().orc_fmm_pair(kernel.h, self.h, _ptr(xs), sources.h, _ptr(xj), _ptr(qj), theta, nspawn,
+                           _ptr(trg))
+        return trg
 
     def fmm(self, kernel, x_sorted, q_sorted, theta=0.4, nspawn=5000, want_ml=False):
         L = lib()

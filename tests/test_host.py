@@ -177,3 +177,41 @@ def test_m2l_tiles_cover_every_pair(tmp_path, n, tt):
                     os.path.join(pkg, "build", "fmm_host.o"), "-o", exe], check=True)
     out = subprocess.run([exe, str(n), str(tt)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
+@pytest.mark.parametrize("dist,n,ncrit,theta,nspawn", [("cube", 8000, 64, 0.4, 5000),
+                                                       ("plummer", 6000, 16, 0.5, 100),
+                                                       ("cube", 120000, 64, 0.4, 5000)])
+def test_pair_plan_matches_oracle(dist, n, ncrit, theta, nspawn):
+    """fmm_plan_build_tree in shared bounds + fmm_plan_pair (traverse(icells,
+    jcells), the -j path of examples/fmm.cxx): the forest holds both trees
+    exactly as the oracle builds them, and the pair lists equal the oracle's
+    two-tree traversal (source cells offset by the target tree's cell count)."""
+    F = _F()
+    xi, qi, xj, qj = O.ine_j_bodies(n, dist=dist)
+    lo, hi = O.bounds(xj, O.bounds(xi))
+    ti, tj = O.Tree(xi, ncrit, (lo, hi)), O.Tree(xj, ncrit, (lo, hi))
+    pi = F.Plan.tree_in_bounds(xi, ncrit, lo, hi)
+    pj = F.Plan.tree_in_bounds(xj, ncrit, lo, hi)
+    for p, t in ((pi, ti), (pj, tj)):
+        tr = p.tree()
+        assert np.array_equal(tr["perm"], t.perm)
+        for k in ("iparent", "ichild", "nchild", "ibody", "nbody"):
+            assert np.array_equal(tr[k], getattr(t, k)), k
+        assert np.array_equal(tr["X"], t.X) and np.array_equal(tr["R"], t.R)
+        assert p.stats()["m2l_pairs"] == 0 and p.stats()["p2p_pairs"] == 0
+    pp = F.Plan.pair(pi, pj, theta, nspawn)
+    s = pp.stats()
+    assert s["n"] == 2 * n and s["ncell"] == ti.ncell + tj.ncell
+    ci, cj, ty = ti.traverse_pair(tj, theta, nspawn)
+    mp, ms, ptr, ps = pp.lists()
+    nci = ti.ncell
+    for typ, (p_, s_) in ((0, (mp, ms)), (1, (ptr, ps))):
+        sel = ty == typ
+        a, b = ci[sel], cj[sel] + nci
+        order = np.argsort(a, kind="stable")
+        optr = np.zeros(nci + tj.ncell + 1, np.int64)
+        optr[1:nci + 1] = np.cumsum(np.bincount(a, minlength=nci))
+        optr[nci + 1:] = optr[nci]
+        assert np.array_equal(p_, optr) and np.array_equal(s_, b[order])
+    assert s["m2l_pairs"] == int((ty == 0).sum()) and s["p2p_pairs"] == int((ty == 1).sum())

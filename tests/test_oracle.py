@@ -287,7 +287,7 @@ def _cell_of_bodies(t):
     return cells, starts + offs
 
 
-def check_tree(x, t, ncrit):
+def check_tree(x, t, ncrit, bnds=None):
     """What fixes BuildTree::buildTree's output (include/build_tree.h:29-209,
     bound_box.h:11-25) without running it: the root box from the drifted
     bounds, octant children at +-R/2, every body inside its cell's box and in
@@ -298,7 +298,7 @@ def check_tree(x, t, ncrit):
     n = len(x)
     xs = x[t.perm]
     assert np.array_equal(np.sort(t.perm), np.arange(n))
-    lo, hi = O.bounds(x)
+    lo, hi = O.bounds(x) if bnds is None else bnds
     X0 = (lo + hi) / 2
     R0 = max(np.max(X0 - lo), np.max(hi - X0)) * 1.00001
     assert np.array_equal(t.X[0], X0) and t.R[0] == R0
@@ -346,25 +346,28 @@ def check_tree(x, t, ncrit):
     assert nxt[0] == t.ncell
 
 
-def _mac(t, a, b, theta, xper):
+def _mac(t, a, b, theta, xper, tj=None):
     """traversal.h:117-119: the pair is far enough for M2L."""
-    d = t.X[a] - t.X[b] - xper
-    return (d * d).sum(1) * theta * theta > (t.R[a] + t.R[b]) ** 2 * (1 - 1e-3)
+    tj = t if tj is None else tj
+    d = t.X[a] - tj.X[b] - xper
+    return (d * d).sum(1) * theta * theta > (t.R[a] + tj.R[b]) ** 2 * (1 - 1e-3)
 
 
-def check_lists(t, ci, cj, ty, theta, xper=np.zeros(3)):
+def check_lists(t, ci, cj, ty, theta, xper=np.zeros(3), tj=None):
     """What fixes the dual traversal's lists (include/traversal.h:90-177)
     without running it: every M2L pair passes the acceptance test and every
     P2P pair is two leaves that fail it; every ordered (target, source) body
     pair is covered exactly once, by a P2P pair of their leaves or an M2L pair
     of an ancestor-or-self of each; and no pair is finer than needed: one of
-    the pairs it can have been split from fails the test."""
-    leaf = t.nchild == 0
+    the pairs it can have been split from fails the test.  tj: a separate
+    source tree (traverse(icells, jcells)); ci indexes t, cj indexes tj."""
+    tj = t if tj is None else tj
+    leaf, leaf_j = t.nchild == 0, tj.nchild == 0
     m2l, p2p = ty == 0, ty == 1
     assert np.all(m2l | p2p)
-    assert np.all(_mac(t, ci[m2l], cj[m2l], theta, xper))
-    assert np.all(leaf[ci[p2p]] & leaf[cj[p2p]])
-    assert not np.any(_mac(t, ci[p2p], cj[p2p], theta, xper))
+    assert np.all(_mac(t, ci[m2l], cj[m2l], theta, xper, tj))
+    assert np.all(leaf[ci[p2p]] & leaf_j[cj[p2p]])
+    assert not np.any(_mac(t, ci[p2p], cj[p2p], theta, xper, tj))
     # exact cover: per target leaf, the source body ranges tile [0, n)
     nc = t.ncell
     order = np.argsort(ci, kind="stable")
@@ -379,20 +382,20 @@ def check_lists(t, ci, cj, ty, theta, xper=np.zeros(3)):
         rows = [src[ptr[a]:ptr[a + 1]] for a in chain]
         # P2P sources only on the leaf itself; ancestors hold M2L sources only
         s = np.concatenate(rows)
-        st, ln = t.ibody[s], t.nbody[s]
+        st, ln = tj.ibody[s], tj.nbody[s]
         o = np.argsort(st, kind="stable")
         st, ln = st[o], ln[o]
-        assert st[0] == 0 and np.all(st[1:] == st[:-1] + ln[:-1]) and st[-1] + ln[-1] == t.n, k
+        assert st[0] == 0 and np.all(st[1:] == st[:-1] + ln[:-1]) and st[-1] + ln[-1] == tj.n, k
         for a in chain[1:]:
             assert np.all(ty[order[ptr[a]:ptr[a + 1]]] == 0)
     # not finer than needed
     root = (ci == 0) & (cj == 0)
     a, b = ci[~root], cj[~root]
-    pa, pb = t.iparent[a], t.iparent[b]
+    pa, pb = t.iparent[a], tj.iparent[b]
     has_pa, has_pb = a != 0, b != 0
-    fail_a = has_pa & ~_mac(t, np.where(has_pa, pa, 0), b, theta, xper)
-    fail_b = has_pb & ~_mac(t, a, np.where(has_pb, pb, 0), theta, xper)
-    fail_ab = has_pa & has_pb & ~_mac(t, np.where(has_pa, pa, 0), np.where(has_pb, pb, 0), theta, xper)
+    fail_a = has_pa & ~_mac(t, np.where(has_pa, pa, 0), b, theta, xper, tj)
+    fail_b = has_pb & ~_mac(t, a, np.where(has_pb, pb, 0), theta, xper, tj)
+    fail_ab = has_pa & has_pb & ~_mac(t, np.where(has_pa, pa, 0), np.where(has_pb, pb, 0), theta, xper, tj)
     assert np.all(fail_a | fail_b | fail_ab)
 
 
@@ -435,6 +438,42 @@ def test_acceptance_test_known_answer(sign):
     assert np.all(ty[cross] == (0 if sign > 0 else 1))
     _, _, ty = t.traverse(math.sqrt(1 / 3) * (1 - 1e-5), 5000)
     assert int((ty == 0).sum()) == 2
+
+
+def _ine_j(n, ncrit, dist="cube"):
+    """examples/fmm.cxx -j: both trees in the bounds of all bodies (:59-62)."""
+    xi, qi, xj, qj = O.ine_j_bodies(n, dist=dist)
+    b = O.bounds(xj, O.bounds(xi))
+    return xi, qi, xj, qj, O.Tree(xi, ncrit, b), O.Tree(xj, ncrit, b)
+
+
+@pytest.mark.parametrize("dist,n,ncrit,theta,nspawn", [("cube", 8000, 64, 0.4, 5000),
+                                                       ("plummer", 6000, 16, 0.5, 100),
+                                                       ("cube", 3000, 8, 0.3, 1)])
+def test_pair_lists_fixed_by_invariants(dist, n, ncrit, theta, nspawn):
+    """traverse(icells, jcells): target bodies against source bodies only,
+    every pair once, by the same acceptance and split rules."""
+    xi, qi, xj, qj, ti, tj = _ine_j(n, ncrit, dist)
+    if n < 4000:
+        b = O.bounds(xj, O.bounds(xi))
+        check_tree(xi, ti, ncrit, b)
+        check_tree(xj, tj, ncrit, b)
+    ci, cj, ty = ti.traverse_pair(tj, theta, nspawn)
+    assert len(ci) > 0
+    check_lists(ti, ci, cj, ty, theta, tj=tj)
+
+
+def test_pair_fmm_converges_to_direct():
+    """The pair solve against the direct sum of the sources at the targets:
+    theta -> 0 makes it P2P only (exact to rounding), and at theta = 0.4 the
+    error falls geometrically with P."""
+    xi, qi, xj, qj, ti, tj = _ine_j(3000, 32)
+    xs, xjs, qjs = xi[ti.perm], xj[tj.perm], qj[tj.perm]
+    k = O.Kernel(4)
+    d = O.direct(k, xs, xjs, qjs)
+    assert _rel(ti.fmm_pair(k, xs, tj, xjs, qjs, 1e-6), d) < 1e-13
+    errs = [_rel(ti.fmm_pair(O.Kernel(P), xs, tj, xjs, qjs, 0.4)[:, 0], d[:, 0]) for P in (4, 8, 12)]
+    assert errs[0] < 1e-2 and errs[1] < errs[0] / 10 and errs[2] < errs[1] / 10, errs
 
 
 def test_invariant_checks_catch_planted_mistakes():
