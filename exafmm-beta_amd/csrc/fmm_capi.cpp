@@ -1384,6 +1384,7 @@ fmmdev::DevCells dev_cells_of(const OpDev &dv, const OpCells &o) {
 // n bodies (SoA) from x (n x 3) and q (may be NULL), targets zeroed
 struct OpBodies {
   size_t x, y, z, q, pot, ax, ay, az;
+  size_t nb;  // bodies in the layout: the stride of every array above
 };
 OpBodies op_bodies(OpLayout &h, int n, const double *x, const double *q) {
   OpBodies o;
@@ -1401,18 +1402,20 @@ OpBodies op_bodies(OpLayout &h, int n, const double *x, const double *q) {
   o.ax = h.zeros(n);
   o.ay = h.zeros(n);
   o.az = h.zeros(n);
+  o.nb = (size_t)n;
   return o;
 }
 fmmdev::DevBodies dev_bodies_of(const OpDev &dv, const OpBodies &o) {
   return {dv.d + o.x, dv.d + o.y, dv.d + o.z, dv.d + o.q, dv.d + o.pot, dv.d + o.ax, dv.d + o.ay, dv.d + o.az};
 }
-// the targets (pot, ax, ay, az) of n bodies added into trg (n x 4)
+// the targets (pot, ax, ay, az) of the first n bodies added into trg (n x 4);
+// the four arrays are contiguous, o.nb values each
 int op_add_targets(fmm_ctx *c, const OpDev &dv, const OpBodies &o, int n, double *trg) {
   std::vector<double> h;
-  int rc = op_download(c, dv, o.pot, 4 * (size_t)n, h);  // pot, ax, ay, az are contiguous
+  int rc = op_download(c, dv, o.pot, 4 * o.nb, h);
   if (rc) return rc;
   for (int b = 0; b < n; b++)
-    for (int k = 0; k < 4; k++) trg[4 * (size_t)b + k] += h[(size_t)k * n + b];
+    for (int k = 0; k < 4; k++) trg[4 * (size_t)b + k] += h[(size_t)k * o.nb + b];
   return FMM_OK;
 }
 // expansions: reference values -> device normalisation (M / s or L * s)

@@ -47,14 +47,21 @@ def test_operators_match_oracle(P):
     M0 = rng.normal(size=k.nterm) + 1j * rng.normal(size=k.nterm)
     M = f.kernel_p2m(Xc, xb, q, M0.copy())
     assert _rel(M - M0, k.p2m(Xc, xb, q)) < OP_TOL
-    # M2M of three children into a parent
-    Xp = Xc + np.array([0.5, -0.5, 0.5])
-    kids = [Xc, Xc + [-1.0, 0, 0], Xc + [0, 1.0, -1.0]]
-    Ms = [k.p2m(c, xb, q) for c in kids]
+    # M2M of three children into a parent, in a tree's geometry (children at
+    # half the parent's radius per axis, bodies within a child's radius): the
+    # oracle's, and the exact identity M2M(P2M about the children) = P2M about
+    # the parent (M2M of degrees below P is exact)
+    Xp = rng.uniform(-1, 1, 3)
+    kids = [Xp + 0.5 * np.array(o) for o in ([1, -1, 1], [-1, -1, -1], [-1, 1, 1])]
+    xbs = [c + rng.uniform(-0.5, 0.5, (37, 3)) for c in kids]
+    qbs = [rng.uniform(-1, 1, 37) for c in kids]
+    Ms = [k.p2m(c, xk, qk) for c, xk, qk in zip(kids, xbs, qbs)]
     ref = np.zeros(k.nterm, complex)
     for c, m in zip(kids, Ms):
         k.m2m(Xp, c, m, ref)
-    assert _rel(f.kernel_m2m(Xp, np.array(kids), np.array(Ms)), ref) < OP_TOL
+    got = f.kernel_m2m(Xp, np.array(kids), np.array(Ms))
+    assert _rel(got, ref) < OP_TOL
+    assert _rel(got, k.p2m(Xp, np.concatenate(xbs), np.concatenate(qbs))) < 1e-13
     # M2L, batched, with and without a periodic shift
     Mp = ref
     Xi = np.array([Xp + [4.0, 1.0, -3.0], Xp + [-5.0, 2.0, 1.0], Xp + [0.5, 6.0, 0.0]])
@@ -96,12 +103,19 @@ def test_kernel_example_matches_fixture():
         assert _rel(rows["direct"], fx["direct_p%d" % P][0]) < 1e-14, P
         first = out.stdout.splitlines()[0].split()
         assert int(first[0]) == P
-        err = np.linalg.norm(fx["fmm_p%d" % P][0][:1] - fx["direct_p%d" % P][0][:1]) / 0.25
+        # Verify::getDifScalar / getNrmScalar(bodies): normalised by the FMM value
+        fm, di = fx["fmm_p%d" % P][0][0], fx["direct_p%d" % P][0][0]
+        err = abs(fm - di) / abs(fm)
         assert abs(float(first[1]) - err) <= 1e-5 * err
 
 
-@pytest.mark.parametrize("dist,n,ncrit,theta,P", [("cube", 20000, 64, 0.4, 8), ("plummer", 8000, 16, 0.5, 6)])
-def test_pair_solve_matches_oracle(dist, n, ncrit, theta, P):
+# the last column bounds the error against the direct sum: the truncation
+# error of the two-tree FMM with zero-mean charges (the oracle's own:
+# cube P = 8, theta = 0.4: 4e-4; Plummer P = 6, theta = 0.5: 1.7e-2 / 2.3e-2,
+# 1.8e-3 / 2.0e-3 at P = 10), with a margin of about 2x
+@pytest.mark.parametrize("dist,n,ncrit,theta,P,bound", [("cube", 20000, 64, 0.4, 8, 1e-3),
+                                                        ("plummer", 8000, 16, 0.5, 6, 5e-2)])
+def test_pair_solve_matches_oracle(dist, n, ncrit, theta, P, bound):
     """Targets against a separate source tree (fmm_plan_pair): every target
     body against the oracle's two-tree FMM at 1e-10; the source bodies get
     nothing."""
@@ -122,7 +136,7 @@ def test_pair_solve_matches_oracle(dist, n, ncrit, theta, P):
     assert np.all(trg[n:] == 0)
     d = O.direct(O.Kernel(P), xi[ti.perm][::97], xj, qj)
     e = O.rel_l2(d, trg[:n][::97])
-    assert e[0] < 1e-3 and e[1] < 1e-3
+    assert e[0] < bound and e[1] < bound, e
 
 
 def test_example_driver_separate_sources():
