@@ -249,6 +249,7 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
         std::lock_guard<std::mutex> lk(g_init_mu);
         if (!g_init_done[p->device]) {
           HIPCHK(fmmdev::init_constants());
+          HIPCHK(fmmdev::init_scratch(p->device));
           g_init_done[p->device] = true;
         }
       }
@@ -1168,35 +1169,27 @@ int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, do
   if (!(cy[0] > 0 && cy[1] > 0 && cy[2] > 0)) FMM_FAIL(FMM_ERR_ARG, "fmm_ewald: cycle must be > 0");
   if (n == 0) return FMM_OK;
   HIPCHK(hipSetDevice(c->p.device));
-  // bodies wrapped into the box (md_distributed.cxx:329), structure of arrays
+  // bodies as structure of arrays; the kernels wrap each position into the
+  // box as they read it (md_distributed.cxx:329)
   std::vector<double> h(4 * (size_t)n);
   for (size_t i = 0; i < (size_t)n; i++) {
-    for (int d = 0; d < 3; d++) {
-      double v = x[3 * i + d];
-      if (v < -cy[d] / 2) v += cy[d];
-      if (v > cy[d] / 2) v -= cy[d];
-      h[(size_t)d * n + i] = v;
-    }
+    for (int d = 0; d < 3; d++) h[(size_t)d * n + i] = x[3 * i + d];
     h[3 * (size_t)n + i] = q[i];
   }
-  // Ewald::initWaves (include/ewald.h:64-88): the half space of k, |k| <= ksize
-  std::vector<int> kv;
-  for (int l = 0; l <= ksize; l++)
-    for (int m = l == 0 ? 0 : -ksize; m <= ksize; m++)
-      for (int k = (l == 0 && m == 0) ? 1 : -ksize; k <= ksize; k++)
-        if (l * l + m * m + k * k <= ksize * ksize) kv.insert(kv.end(), {l, m, k});
-  const int nw = (int)(kv.size() / 3);
+  // Ewald::initWaves (include/ewald.h:64-88): the half space of k, |k| <= ksize,
+  // tabulated on the device
+  const int nw = fmmdev::ewald_nwaves(ksize);
   int rc;
   if ((rc = upload(c, c->ew_b, h.data(), h.size()))) return rc;
   if ((rc = ensure(c, c->ew_t, sizeof(double) * 4 * n))) return rc;
-  if ((rc = upload(c, c->ew_k, kv.data(), kv.size()))) return rc;
+  if ((rc = ensure(c, c->ew_k, sizeof(int) * 3 * std::max(nw, 1)))) return rc;
   if ((rc = ensure(c, c->ew_s, sizeof(double2) * std::max(nw, 1)))) return rc;
   HIPCHK(hipMemsetAsync(c->ew_t.p, 0, sizeof(double) * 4 * n, c->st));
   const double *bx = c->ew_b.as<double>();
   double *tx = c->ew_t.as<double>();
-  const fmmdev::DevBodies B{bx, bx + n, bx + 2 * (size_t)n, bx + 3 * (size_t)n,
-                            tx, tx + n, tx + 2 * (size_t)n, tx + 3 * (size_t)n};
-  HIPCHK(fmmdev::launch_ewald(B, n, c->ew_k.as<int>(), nw, c->ew_s.as<double2>(),
+  const fmmdev::EwBodies B{bx, bx + n, bx + 2 * (size_t)n, bx + 3 * (size_t)n,
+                           tx, tx + n, tx + 2 * (size_t)n, tx + 3 * (size_t)n, 1, 1};
+  HIPCHK(fmmdev::launch_ewald(B, n, ksize, c->ew_k.as<int>(), c->ew_s.as<double2>(),
                               make_double3(cy[0], cy[1], cy[2]), alpha, sigma, cutoff, c->st));
   std::vector<double> o(4 * (size_t)n);
   HIPCHK(hipMemcpyAsync(o.data(), c->ew_t.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->st));
@@ -1205,6 +1198,26 @@ int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, do
     p[i] += o[i];
     for (int d = 0; d < 3; d++) f[3 * i + d] += o[(size_t)(d + 1) * n + i];
   }
+  return FMM_OK;
+}
+
+int fmm_ewald_noalloc(fmm_ctx *c, int n, const double *x, const double *q, int ksize, double alpha, double sigma,
+                      double cutoff, double *p, double *f) {
+  if (!c) return FMM_ERR_ARG;
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  if (n < 0 || (n > 0 && (!x || !q || !p || !f)) || ksize < 0 || !(alpha > 0) || !(sigma > 0) || !(cutoff >= 0))
+    FMM_FAIL(FMM_ERR_ARG, "fmm_ewald_noalloc: bad arguments");
+  if (ksize > fmmdev::kEwaldScratchMaxK)
+    FMM_FAIL(FMM_ERR_ARG, "fmm_ewald_noalloc: ksize " + std::to_string(ksize) + " exceeds the scratch limit of " +
+                              std::to_string(fmmdev::kEwaldScratchMaxK));
+  const double *cy = c->p.cycle;
+  if (!(cy[0] > 0 && cy[1] > 0 && cy[2] > 0)) FMM_FAIL(FMM_ERR_ARG, "fmm_ewald_noalloc: cycle must be > 0");
+  if (n == 0) return FMM_OK;
+  HIPCHK(hipSetDevice(c->p.device));
+  // the caller's n x 3 positions and forces read in place (stride 3)
+  const fmmdev::EwBodies B{x, x + 1, x + 2, q, p, f, f + 1, f + 2, 3, 3};
+  HIPCHK(fmmdev::launch_ewald_scratch(c->p.device, B, n, ksize, make_double3(cy[0], cy[1], cy[2]), alpha, sigma,
+                                      cutoff, c->st));
   return FMM_OK;
 }
 

@@ -113,12 +113,29 @@ hipError_t launch_p2p(DevBodies B, DevCells C, const int *targets, int ntarget, 
 // getDipole + dipoleCorrection on the bodies' targets; part: 3 * dipole_partials(n) doubles
 int dipole_partials(int n);
 hipError_t launch_dipole_correction(DevBodies B, int n, double *part, double coef, hipStream_t s);
-// Ewald sum (include/ewald.h) over the bodies B (positions inside the box):
-// wave part over the nw wave vectors kv (3 ints each; S: nw scratch values),
-// real part over minimum-image pairs within the cutoff, self term; adds into
-// B's targets
-hipError_t launch_ewald(DevBodies B, int n, const int *kv, int nw, double2 *S, double3 cyc, double alpha,
+// Ewald sum (include/ewald.h): bodies with positions x[i*xs], y[i*xs],
+// z[i*xs] (each wrapped once into the box as it is read), charges q[i]; the
+// results are added into pot[i] and ax/ay/az[i*fs]
+struct EwBodies {
+  const double *x, *y, *z, *q;
+  double *pot, *ax, *ay, *az;
+  int xs, fs;
+};
+// number of wave vectors of Ewald::initWaves for ksize
+int ewald_nwaves(int ksize);
+// wave part over the ewald_nwaves(ksize) wave vectors (table kv: 3 ints each,
+// S: one double2 each, both written here), real part over minimum-image pairs
+// within the cutoff, self term
+hipError_t launch_ewald(EwBodies B, int n, int ksize, int *kv, double2 *S, double3 cyc, double alpha,
                         double sigma, double cutoff, hipStream_t s);
+// The same on the device's static scratch array (no allocation): ksize up to
+// kEwaldScratchMaxK; init_scratch(device) must have run once with that device
+// current.  Calls that overlap on one device share the array and race.
+constexpr int kMaxDevices = 64;
+constexpr int kEwaldScratchMaxK = 32;
+hipError_t init_scratch(int device);
+hipError_t launch_ewald_scratch(int device, EwBodies B, int n, int ksize, double3 cyc, double alpha, double sigma,
+                                double cutoff, hipStream_t s);
 hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,
                          double *out, double eps2, int prange, double3 cyc, hipStream_t s);
 

@@ -137,6 +137,8 @@ def load():
         L.fmm_md_evaluate.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.fmm_ewald.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, _dp, _dp]
+        L.fmm_ewald_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, vp, vp]
         _up = ctypes.POINTER(ctypes.c_uint64)
         L.fmm_plan_list_counts.argtypes = [vp, _up, _ip, _ip]
         L.fmm_get_list_counts.argtypes = [vp, _up, _ip, _ip]
@@ -327,6 +329,14 @@ class FMM:
         self._check(load().fmm_ewald(self._h, n, _ptr(x), _ptr(q), ksize, alpha, sigma, cutoff, _ptr(p),
                                      _ptr(f)), "fmm_ewald")
         return p, f
+
+    def ewald_noalloc(self, n, x_dev, q_dev, ksize, alpha, sigma, cutoff, p_dev, f_dev):
+        """fmm_ewald_noalloc: the Ewald sum on device arrays given by address
+        (x n x 3, q n, p n, f n x 3, float64, on this context's device); adds
+        into p and f on the context's stream without waiting and allocates
+        nothing (ksize <= 32)."""
+        self._check(load().fmm_ewald_noalloc(self._h, int(n), x_dev, q_dev, int(ksize), alpha, sigma, cutoff,
+                                             p_dev, f_dev), "fmm_ewald_noalloc")
 
     def stats(self):
         s = Stats()

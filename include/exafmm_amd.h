@@ -233,6 +233,19 @@ int fmm_md_evaluate(fmm_ctx *c, int n, const double *x, const double *q, double 
 int fmm_ewald(fmm_ctx *c, int n, const double *x, const double *q, int ksize, double alpha, double sigma,
               double cutoff, double *p, double *f);
 
+/* No counterpart in the reference: fmm_ewald for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  x (n x 3), q (n), p (n) and f (n x 3) are DEVICE
+ * pointers of the context's device; p[i] += potential, f[3i..3i+2] +=
+ * gradient, with the same kernels and summation order as fmm_ewald.  The wave
+ * table and structure factors live in a static per-device scratch array of
+ * about 1.9 MB, so ksize <= 32 (larger ksize: FMM_ERR_ARG).  Enqueued on the
+ * context's stream and returns without waiting (fmm_synchronize waits);
+ * calls no hipMalloc/hipFree.  Calls that overlap on one device share the
+ * scratch array and race: serialise them (one stream, or events). */
+int fmm_ewald_noalloc(fmm_ctx *c, int n, const double *x, const double *q, int ksize, double alpha, double sigma,
+                      double cutoff, double *p, double *f);
+
 /* Traversal::direct (include/traversal.h:355-377): direct sum over all
  * bodies of the context (tree order) -- with images > 0 over the cube of
  * shifts (ix, iy, iz) * cycle, |i| <= sum_(l < images) neighbor^l -- at nt
