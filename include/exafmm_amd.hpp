@@ -64,6 +64,81 @@ template <int N> struct vec {
 typedef vec<3> vec3;
 typedef vec<4> vec4;
 
+// Compensated (Kahan) summation, include/kahan.h:9-124: `s` is the running
+// sum and `c` the rounding error of the last additions (the part of the true
+// sum that `s` lost, with its sign reversed), so the value is s - c.  Same
+// members and operators as the reference's kahan<T>; its conversion and
+// stream output use s - c (the reference's operator T() returns s + c, which
+// adds the lost part with the wrong sign), and adding a kahan<T> adds its
+// value s - c.  The device kernels' compensated accumulation (mixed
+// precision P2P) follows the same recurrence.
+template <typename T> struct kahan {
+  T s, c;
+  kahan() {}
+  kahan(const T &v) : s(v), c(0) {}
+  kahan(const kahan &v) : s(v.s), c(v.c) {}
+  kahan &operator=(const T v) {
+    s = v;
+    c = 0;
+    return *this;
+  }
+  kahan &operator=(const kahan &v) {
+    s = v.s;
+    c = v.c;
+    return *this;
+  }
+  kahan &operator+=(const T v) {  // Kahan's recurrence: y = v - c, t = s + y, c = (t - s) - y
+    const T y = v - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+    return *this;
+  }
+  kahan &operator-=(const T v) { return *this += -v; }
+  kahan &operator*=(const T v) {
+    s *= v;
+    c *= v;
+    return *this;
+  }
+  kahan &operator/=(const T v) {
+    s /= v;
+    c /= v;
+    return *this;
+  }
+  kahan &operator+=(const kahan &v) {
+    *this += v.s;
+    return *this += -v.c;
+  }
+  kahan &operator-=(const kahan &v) {
+    *this += -v.s;
+    return *this += v.c;
+  }
+  kahan &operator*=(const kahan &v) { return *this *= T(v); }
+  kahan &operator/=(const kahan &v) { return *this /= T(v); }
+  kahan operator-() const {
+    kahan t;
+    t.s = -s;
+    t.c = -c;
+    return t;
+  }
+  operator T() const { return s - c; }
+  friend std::ostream &operator<<(std::ostream &o, const kahan &v) { return o << (v.s - v.c); }
+  friend std::istream &operator>>(std::istream &i, kahan &v) {
+    i >> v.s;
+    v.c = 0;
+    return i;
+  }
+};
+// include/types.h:35-42: kreal_t / kcomplex_t are the compensated types when
+// EXAFMM_USE_KAHAN is set, the plain ones otherwise
+#if defined(EXAFMM_USE_KAHAN) && EXAFMM_USE_KAHAN
+typedef kahan<real_t> kreal_t;
+typedef kahan<complex_t> kcomplex_t;
+#else
+typedef real_t kreal_t;
+typedef complex_t kcomplex_t;
+#endif
+
 //! include/types.h:72-84
 struct Body {
   vec3 X;

@@ -276,3 +276,30 @@ def test_plan_list_counts_match_oracle():
         oct_ = sum(1 << d for d in range(3) if tr["X"][cc][d] > tr["X"][p][d])
         assert (int(key[cc]) - off(lc)) >> 3 == int(key[p]) - off(lc - 1)
         assert (int(key[cc]) - off(lc)) & 7 == oct_
+
+
+def test_kahan_compensates():
+    """kahan<T> (include/kahan.h) in the C++ layer: 1 + 2^20 terms each below
+    half an ulp of the sum.  The plain sum never moves; the compensated value
+    is the exact sum to a few ulps (float: 1 + 2^20 * 1e-8 = 1.01048576, the
+    terms being the float nearest 1e-8), and the compensation term is non-zero."""
+    exe = os.path.join(os.environ.get("TMPDIR", "/tmp"), "kahan_check_%d" % os.getpid())
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "kahan_check.cpp"), "-o", exe], check=True)
+    out = {l.split()[0]: [float(v) for v in l.split()[1:]]
+           for l in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines()}
+    os.unlink(exe)
+    small32 = float(np.float32(1e-8))
+    plain, val, s, c = out["float"]
+    exact = 1.0 + (1 << 20) * small32
+    assert plain == 1.0
+    assert abs(val - exact) <= 4 * np.spacing(np.float32(exact)) and c != 0.0
+    plain, val, s, c = out["double"]
+    exact = 1.0 + (1 << 20) * 1e-17
+    assert plain == 1.0
+    assert abs(val - exact) <= 4 * np.spacing(exact) and c != 0.0
+    s_val, m_val, neg, b = out["ops"]
+    assert abs(s_val - 1.0) <= 2 * np.spacing(1.0)
+    assert abs(m_val - (1.0 + 1e-14) / 2) <= 2 * np.spacing(0.5)
+    assert abs(neg + 1.0 + 1e-14) <= 2 * np.spacing(1.0)
+    assert abs(b + 1e-14) <= 1e-14 * 1e-10
