@@ -63,6 +63,10 @@ struct fmm_ctx {
   hipStream_t st2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   bool timing = false;     // set by fmm_evaluate
+  // fmm_evaluate_graph: the solve captured from both streams, replayed until
+  // the bodies, tree or any device buffer of the solve change
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   fmm::Tree tree;
   fmm::Lists lists;
   int n = 0;
@@ -157,6 +161,14 @@ void release(DBuf &b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
+}
+
+// drop the captured solve (its nodes point at the device buffers as they were)
+void graph_reset(fmm_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
 }
 
 fmmdev::DevBodies dev_bodies(fmm_ctx *c) {
@@ -270,6 +282,7 @@ void fmm_destroy(fmm_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->p.device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  graph_reset(c);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
@@ -577,6 +590,7 @@ int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const d
 namespace {
 int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   HIPCHK(hipSetDevice(c->p.device));
+  graph_reset(c);
   c->have_bodies = false;
   const double t3 = now_ms();
   const int n = pl.n;
@@ -914,7 +928,9 @@ int fmm_set_charges(fmm_ctx *c, const double *q) {
   if (!q && c->n > 0) FMM_FAIL(FMM_ERR_ARG, "fmm_set_charges: NULL q");
   std::vector<double> hq(c->n);
   for (int i = 0; i < c->n; i++) hq[i] = q[c->tree.perm[i]];
+  const void *before = c->q.p;
   if ((rc = upload(c, c->q, hq.data(), hq.size()))) return rc;
+  if (c->q.p != before) graph_reset(c);  // a captured solve reads the charges in place
   HIPCHK(hipStreamSynchronize(c->st));
   return FMM_OK;
 }
@@ -942,12 +958,11 @@ int fmm_upward(fmm_ctx *c) {
 namespace {
 // P2P (targets overwritten) on stream s
 int p2p_on(fmm_ctx *c, hipStream_t s) {
-  const size_t bytes = sizeof(double) * c->n;
   if (c->n > 0) {
-    HIPCHK(hipMemsetAsync(c->pot.p, 0, bytes, s));
-    HIPCHK(hipMemsetAsync(c->ax.p, 0, bytes, s));
-    HIPCHK(hipMemsetAsync(c->ay.p, 0, bytes, s));
-    HIPCHK(hipMemsetAsync(c->az.p, 0, bytes, s));
+    HIPCHK(fmmdev::launch_zero(c->pot.as<double>(), c->n, s));
+    HIPCHK(fmmdev::launch_zero(c->ax.as<double>(), c->n, s));
+    HIPCHK(fmmdev::launch_zero(c->ay.as<double>(), c->n, s));
+    HIPCHK(fmmdev::launch_zero(c->az.as<double>(), c->n, s));
   }
   const double3 xper = c->cyc;  // image shifts of the source codes
   if (c->timing) HIPCHK(hipEventRecord(c->kev[2], s));
@@ -969,7 +984,7 @@ int fmm_m2l(fmm_ctx *c) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (c->tree.ncell == 0) return FMM_OK;
-  HIPCHK(hipMemsetAsync(c->L.p, 0, sizeof(double2) * nterm(c) * c->tree.ncell, c->st));
+  HIPCHK(fmmdev::launch_zero(c->L.as<double>(), 2 * (size_t)nterm(c) * c->tree.ncell, c->st));
   const double3 xper = c->cyc;  // image shifts of the source codes
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   if (c->use_mfma) {
@@ -1032,29 +1047,39 @@ int fmm_downward(fmm_ctx *c) {
   return FMM_OK;
 }
 
-int fmm_evaluate(fmm_ctx *c) {
-  int rc = check_ready(c);
-  if (rc) return rc;
-  c->timing = true;
-  rc = [&]() -> int {
-  // P2P needs the bodies only: it runs on the second stream beside the
-  // upward pass and M2L (vector ALUs beside the matrix cores), and L2P, which
-  // adds into the same targets, waits for it
-  // (forking after the upward pass instead, so that P2P runs beside M2L, was
-  // measured no faster: M2L's LDS leaves no room for P2P blocks on a CU)
-  HIPCHK(hipEventRecord(c->ev[0], c->st));
+namespace {
+// The solve's launch sequence on the context's two streams: P2P needs the
+// bodies only, so it runs on the second stream beside the upward pass and M2L
+// (vector ALUs beside the matrix cores), and L2P, which adds into the same
+// targets, waits for it (forking after the upward pass instead, so that P2P
+// runs beside M2L, was measured no faster: M2L's LDS leaves no room for P2P
+// blocks on a CU).  With c->timing the phase events are recorded too.
+int solve_sequence(fmm_ctx *c) {
+  int rc;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], c->st));
   HIPCHK(hipEventRecord(c->fork, c->st));
   HIPCHK(hipStreamWaitEvent(c->st2, c->fork, 0));
   if ((rc = p2p_on(c, c->st2))) return rc;
   HIPCHK(hipEventRecord(c->join, c->st2));
   if ((rc = fmm_upward(c))) return rc;
-  HIPCHK(hipEventRecord(c->ev[1], c->st));
-  HIPCHK(hipEventRecord(c->ev[2], c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->st));
   if ((rc = fmm_m2l(c))) return rc;
-  HIPCHK(hipEventRecord(c->ev[3], c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->st));
   HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   if ((rc = fmm_downward(c))) return rc;
-  HIPCHK(hipEventRecord(c->ev[4], c->st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
+  return FMM_OK;
+}
+}  // namespace
+
+int fmm_evaluate(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  c->timing = true;
+  rc = [&]() -> int {
+  int rc2 = solve_sequence(c);
+  if (rc2) return rc2;
   HIPCHK(hipEventSynchronize(c->ev[4]));
   float ms[4];
   for (int i = 0; i < 4; i++) HIPCHK(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
@@ -1076,6 +1101,42 @@ int fmm_evaluate(fmm_ctx *c) {
   }();
   c->timing = false;
   return rc;
+}
+
+int fmm_evaluate_graph(fmm_ctx *c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->gexec) {
+    // capture the solve from the main stream (the second stream joins through
+    // the fork event); kernel nodes only: the phases clear with kernels
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    rc = solve_sequence(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc) return rc;
+      c->err = std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e);
+      return FMM_ERR_DEVICE;
+    }
+    c->graph = g;
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      graph_reset(c);
+      c->err = std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei);
+      return FMM_ERR_DEVICE;
+    }
+  }
+  HIPCHK(hipEventRecord(c->ev[0], c->st));
+  HIPCHK(hipGraphLaunch(c->gexec, c->st));
+  HIPCHK(hipEventRecord(c->ev[4], c->st));
+  HIPCHK(hipEventSynchronize(c->ev[4]));
+  float tot;
+  HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[4]));
+  c->stats.dev_upward_ms = c->stats.dev_m2l_ms = c->stats.dev_downward_ms = 0.0;
+  c->stats.dev_p2p_ms = c->stats.dev_m2l_kernel_ms = c->stats.dev_p2p_kernel_ms = 0.0;
+  c->stats.dev_total_ms = tot;
+  return FMM_OK;
 }
 
 int fmm_direct(fmm_ctx *c, int nt, const double *xt, double *trg) {

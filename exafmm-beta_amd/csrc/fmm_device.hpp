@@ -25,6 +25,9 @@ struct DevCells {
 
 constexpr int kMaxP = 20;
 
+// p[0, n) = 0 with a kernel (no memset node when the stream is being captured)
+hipError_t launch_zero(double *p, size_t n, hipStream_t s);
+
 bool order_supported(int P);
 hipError_t init_constants();
 

@@ -30,7 +30,7 @@ SYMBOLS = [
     "fmm_server_get_stats", "fmm_server_destroy", "fmm_plan_prepare",
     "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts", "fmm_ewald",
     "fmm_plan_build_tree", "fmm_plan_pair", "fmm_kernel_p2p", "fmm_kernel_p2m", "fmm_kernel_m2m",
-    "fmm_kernel_m2l", "fmm_kernel_l2l", "fmm_kernel_l2p",
+    "fmm_kernel_m2l", "fmm_kernel_l2l", "fmm_kernel_l2p", "fmm_ewald_noalloc", "fmm_evaluate_graph",
 ]
 
 FMM_FP64 = 0
@@ -117,7 +117,7 @@ def load():
         L.fmm_last_error.argtypes = [vp]
         L.fmm_set_bodies.argtypes = [vp, ctypes.c_int, _dp, _dp]
         L.fmm_set_charges.argtypes = [vp, _dp]
-        for f in ("fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
+        for f in ("fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate", "fmm_evaluate_graph",
                   "fmm_synchronize", "fmm_dipole_correction"):
             getattr(L, f).argtypes = [vp]
         L.fmm_get_targets.argtypes = [vp, _dp, ctypes.c_int]
@@ -278,8 +278,13 @@ class FMM:
     def downward(self):
         self._check(load().fmm_downward(self._h), "fmm_downward")
 
-    def evaluate(self):
-        self._check(load().fmm_evaluate(self._h), "fmm_evaluate")
+    def evaluate(self, graph=False):
+        """one solve; graph=True replays the solve captured as a HIP graph
+        (captured on the first such call after the bodies change)"""
+        if graph:
+            self._check(load().fmm_evaluate_graph(self._h), "fmm_evaluate_graph")
+        else:
+            self._check(load().fmm_evaluate(self._h), "fmm_evaluate")
 
     def synchronize(self):
         self._check(load().fmm_synchronize(self._h), "fmm_synchronize")

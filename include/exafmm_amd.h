@@ -205,6 +205,13 @@ int fmm_m2l(fmm_ctx *c);
 int fmm_downward(fmm_ctx *c);
 /* upward + p2p + m2l + downward, with per-phase device timing */
 int fmm_evaluate(fmm_ctx *c);
+/* No counterpart in the reference: fmm_evaluate as one HIP graph launch.  The
+ * first call after the bodies, tree or charges buffer change captures the
+ * solve's launch sequence (both streams, kernel nodes only) and instantiates
+ * it; later calls replay it (for repeated solves of one system with new
+ * charges, fmm_set_charges).  Same kernels and results as fmm_evaluate;
+ * fmm_stats reports only dev_total_ms for it. */
+int fmm_evaluate_graph(fmm_ctx *c);
 int fmm_synchronize(fmm_ctx *c);
 
 /* UpDownPass::getDipole(bodies, X0 = 0) + dipoleCorrection(bodies, dipole, n,
