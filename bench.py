@@ -346,6 +346,38 @@ def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0, 
     }
 
 
+def graph_small(F, log, steps=50):
+    """Serving-sized systems (N = 2^14 and 2^16 uniform cube, P = 8, mixed):
+    per-solve latency of the eager launch sequence against one replay of the
+    solve captured as a HIP graph (fmm_evaluate_graph), same system."""
+    out = {}
+    for lg in (14, 16):
+        n = 1 << lg
+        x, q = F.make_bodies(n, "cube1", lg)
+        f = F.FMM(P=8, precision=F.FMM_MIXED)
+        f.set_bodies(x, q)
+        res = {}
+        for mode in ("eager", "graph"):
+            for _ in range(3):
+                f.evaluate(graph=mode == "graph")
+            lat = []
+            for _ in range(steps):
+                a = time.perf_counter()
+                f.evaluate(graph=mode == "graph")
+                lat.append(time.perf_counter() - a)
+            res[mode] = {"p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                         "device_ms": f.stats()["dev_total_ms"]}
+            res[mode + "_targets"] = f.targets(input_order=False)
+        d = rel_l2(res.pop("eager_targets"), res.pop("graph_targets"))
+        res["graph_vs_eager_rel_l2"] = {"pot": d[0], "acc": d[1]}
+        f.close()
+        log("[bench] graph N=2^%d: eager p50 %.3f ms, graph p50 %.3f ms" % (lg, res["eager"]["p50_ms"],
+                                                                            res["graph"]["p50_ms"]))
+        out["n=2^%d" % lg] = res
+    return out
+
+
 def serve_bench(F, name, nreq, check_every, log):
     """Poisson arrivals into the request loop; latency = submit -> result on
     the server's clock; throughput over the whole stream; rel-L2 vs the FP64
@@ -431,6 +463,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy-point", action="store_true",
                     help="skip the smaller-theta solve that reaches rel-L2 <= 1e-6")
+    ap.add_argument("--no-graph-small", action="store_true",
+                    help="skip the eager-vs-graph latency of serving-sized systems")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter calibration)")
@@ -556,6 +590,8 @@ def main():
         pts["mixed_over_fp64"] = pts["mixed"]["value"] / pts["fp64"]["value"]
         out["accuracy_point"] = pts
         del plan2
+    if not args.no_graph_small:
+        out["graph_small"] = graph_small(F, log)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, q, w, plan, args.cpu_budget)
     print(json.dumps(out))

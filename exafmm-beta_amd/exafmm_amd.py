@@ -35,7 +35,7 @@ SYMBOLS = [
 
 FMM_FP64 = 0
 FMM_MIXED = 1
-M2L_AUTO = 0      # offset groups on MFMA: target tiles (FP64 P <= 10, mixed P <= 12), else chunks
+M2L_AUTO = 0      # offset groups on MFMA in target tiles (P <= 12), else per target
 M2L_TARGET = 1    # per-target kernel on vector ALUs, fixed summation order
 M2L_GROUPED = 2   # offset groups on MFMA, one block per group chunk, FP64 atomics
 M2L_TILES = 3     # offset groups on MFMA, one block per target tile, LDS accumulation

@@ -44,9 +44,8 @@ typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
  * target cells and sums all their groups in LDS before one store (a tile too
  * long for one block is split and its parts are added atomically).  Chunks:
  * one block per chunk of <= 512 pairs of one group, results added with FP64
- * atomics.  AUTO (P <= 12) uses tiles for FP64 P <= 10 and every mixed order,
- * chunks for FP64 P = 11, 12 (measured: the atomics cost more than the
- * tiles' operator reads except for the FP64 tables of the highest orders); GROUPED
+ * atomics.  AUTO (P <= 12) uses tiles (measured faster at every order and
+ * precision: the chunks' atomics cost more than the tiles' operator reads); GROUPED
  * always uses chunks, TILES always tiles; all repeat to rounding.  The m2l_path
  * of fmm_stats reports TILES or GROUPED for the layout taken.  TARGET: one wavefront per
  * target cell on vector ALUs (fixed summation order, bitwise repeatable; any
