@@ -35,6 +35,7 @@ METRIC = "particles/s per FMM solve + p50/p99 latency, mixed vs FP64 at equal L2
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 matrix (MFMA)
 PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X FP32 vector (the packed-FP32 rate)
+PEAK_FP16_TFLOPS = 2500.0  # MI355X FP16 dense MFMA
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
@@ -294,8 +295,9 @@ def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0, 
     """Upload one system (host plan shared), run `warmup` + `steps` solves, and
     return timings, kernel figures and the accuracy on sampled targets."""
     n, P = w["n"], w["P"]
+    prec = {"fp64": F.FMM_FP64, "mixed": F.FMM_MIXED, "mixed16": F.FMM_MIXED16}[precision]
     f = F.FMM(P=P, theta=theta or w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], m2l_mode=m2l_mode,
-              precision=F.FMM_MIXED if precision == "mixed" else F.FMM_FP64)
+              precision=prec)
     f.set_bodies_plan(plan, q)
     for _ in range(warmup):
         f.evaluate()
@@ -328,7 +330,10 @@ def solve_and_time(F, w, plan, x, q, precision, steps, warmup, log, m2l_mode=0, 
     fm = m2l_flops(P, st["m2l_pairs"])
     fe = m2l_exec_flops(P, st["m2l_pairs"])
     fp = p2p_flops(st["p2p_body_pairs"])
-    peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
+    # mixed16: three FP16 products (hi x hi, hi x lo, lo x hi) per dense term
+    peak = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "mixed16": PEAK_FP16_TFLOPS}[precision]
+    if precision == "mixed16":
+        fe = 3 * fe
     return {
         "wall_s": wall, "lat": lat, "dev": dev, "st": st, "trg": trg, "perm": perm,
         "accuracy": {"pot_rel_l2": pot_rel, "acc_rel_l2": acc_rel, "targets": int(w["ntarget"]),
@@ -463,6 +468,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy-point", action="store_true",
                     help="skip the smaller-theta solve that reaches rel-L2 <= 1e-6")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the 2048-request serving streams at 200 and 1000 req/s")
     ap.add_argument("--no-graph-small", action="store_true",
                     help="skip the eager-vs-graph latency of serving-sized systems")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
@@ -574,6 +581,21 @@ def main():
                          "frac_executed": rm["kernels"]["m2l"]["frac_executed"]}}
         out["mixed_over_fp64"] = vm / value
         out["mixed_vs_fp64_rel_l2"] = dict(zip(("pot", "acc"), rel_l2(r["trg"], rm["trg"])))
+        # the same with the M2L on split-FP16 operands (FMM_MIXED16)
+        rh = solve_and_time(F, w, plan, x, q, "mixed16", args.steps, args.warmup, log, args.m2l_mode)
+        vh = n * args.steps / rh["wall_s"]
+        out["mixed16_same_system"] = {
+            "value": vh, "ms_per_step": 1e3 * rh["wall_s"] / args.steps,
+            "latency_ms": {"p50": 1e3 * float(np.percentile(rh["lat"], 50)),
+                           "p99": 1e3 * float(np.percentile(rh["lat"], 99))},
+            "device_ms": float(np.mean(rh["dev"])), "accuracy": rh["accuracy"], "kernels": rh["kernels"],
+            "roofline": {"bound": "mfma", "kernel": "m2l", "achieved": rh["kernels"]["m2l"]["tflops"],
+                         "peak": rh["kernels"]["m2l"]["peak_tflops"], "unit": "TFLOP/s",
+                         "frac": rh["kernels"]["m2l"]["frac"],
+                         "achieved_executed": rh["kernels"]["m2l"]["exec_tflops"],
+                         "frac_executed": rh["kernels"]["m2l"]["frac_executed"]},
+            "vs_fp64_rel_l2": dict(zip(("pot", "acc"), rel_l2(r["trg"], rh["trg"])))}
+        out["mixed16_over_fp64"] = vh / value
     at = w.get("accurate_theta")
     if at and not args.no_accuracy_point:
         # the same system at a smaller theta, where both precisions reach
@@ -581,17 +603,22 @@ def main():
         plan2 = F.Plan(x, ncrit=w["ncrit"], theta=at, nspawn=w["nspawn"])
         k = max(2, args.steps // 3)
         pts = {}
-        for prec in ("fp64", "mixed"):
+        for prec in ("fp64", "mixed", "mixed16"):
             ra = solve_and_time(F, w, plan2, x, q, prec, k, 1, log, args.m2l_mode, theta=at)
             pts[prec] = {"value": n * k / ra["wall_s"], "ms_per_step": 1e3 * ra["wall_s"] / k,
                          "accuracy": ra["accuracy"], "m2l_pairs": ra["st"]["m2l_pairs"],
                          "p2p_body_pairs": ra["st"]["p2p_body_pairs"]}
         pts["theta"] = at
         pts["mixed_over_fp64"] = pts["mixed"]["value"] / pts["fp64"]["value"]
+        pts["mixed16_over_fp64"] = pts["mixed16"]["value"] / pts["fp64"]["value"]
         out["accuracy_point"] = pts
         del plan2
     if not args.no_graph_small:
         out["graph_small"] = graph_small(F, log)
+    if not args.no_serving:
+        # the serving stream at its stated size: 2048 requests, N = 2^U[12,18]
+        # cube or Plummer, P = 8, mixed, Poisson arrivals at 200 and 1000 req/s
+        out["serving"] = {name: serve_bench(F, name, 0, 64, log) for name in ("serve-200", "serve-1000")}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, q, w, plan, args.cpu_budget)
     print(json.dumps(out))

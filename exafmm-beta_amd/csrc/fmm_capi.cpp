@@ -98,7 +98,8 @@ struct fmm_ctx {
   DBuf per_F, per_T, per_S, per_cur, per_tmp, per_dx, per_kind;
   // mixed precision: per-group (R_s, R_t), cell radii, FP32 multipoles
   bool mixed = false;
-  DBuf g_sc, cr, M32;
+  bool h16 = false;  // FMM_MIXED16: split-FP16 M2L operands (M16 instead of M32)
+  DBuf g_sc, cr, M32, M16;
   DBuf dip;  // dipole partial sums
   DBuf ew_b, ew_t, ew_k, ew_s;  // fmm_ewald: bodies, targets, wave vectors, structure factors
   DBuf op_d, op_i;  // fmm_kernel_*: doubles (bodies, centres, expansions) and ints of one call
@@ -235,8 +236,12 @@ int fmm_create(fmm_ctx **out, const fmm_params *p) {
              p->m2l_mode != FMM_M2L_TILES) {
     c->err = "invalid m2l_mode";
     rc = FMM_ERR_ARG;
-  } else if (p->precision != FMM_FP64 && p->precision != FMM_MIXED) {
+  } else if (p->precision != FMM_FP64 && p->precision != FMM_MIXED && p->precision != FMM_MIXED16) {
     c->err = "unsupported precision mode";
+    rc = FMM_ERR_UNSUPPORTED;
+  } else if (p->precision == FMM_MIXED16 &&
+             (!fmmdev::m2l_mixed_supported(p->P) || p->m2l_mode == FMM_M2L_TARGET || p->m2l_mode == FMM_M2L_GROUPED)) {
+    c->err = "mixed precision with split-FP16 M2L needs 2 <= P <= 12 and m2l_mode AUTO or TILES";
     rc = FMM_ERR_UNSUPPORTED;
   } else if (p->precision == FMM_MIXED &&
              (!fmmdev::m2l_mixed_supported(p->P) || p->m2l_mode == FMM_M2L_TARGET)) {
@@ -428,7 +433,8 @@ int fmm_plan_pair(fmm_plan **out, const fmm_plan *targets, const fmm_plan *sourc
 }
 
 int fmm_plan_build_groups(fmm_plan *pl, int precision) {
-  if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED)) return FMM_ERR_ARG;
+  if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED && precision != FMM_MIXED16)) return FMM_ERR_ARG;
+  if (precision == FMM_MIXED16) precision = FMM_MIXED;  // the same groups (by level)
   try {
     pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED,
                                        pl->images > 0 ? pl->cycle : nullptr);
@@ -441,15 +447,17 @@ int fmm_plan_build_groups(fmm_plan *pl, int precision) {
 }
 
 int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
-  if (!pl || !p || (p->precision != FMM_FP64 && p->precision != FMM_MIXED)) return FMM_ERR_ARG;
+  if (!pl || !p || (p->precision != FMM_FP64 && p->precision != FMM_MIXED && p->precision != FMM_MIXED16))
+    return FMM_ERR_ARG;
   if (p->m2l_mode == FMM_M2L_TARGET) return FMM_OK;
-  const bool mixed = p->precision == FMM_MIXED;
+  const bool mixed = p->precision != FMM_FP64;
+  const int kind = mixed ? FMM_MIXED : FMM_FP64;  // FMM_MIXED16 groups as FMM_MIXED
   if (!(mixed ? fmmdev::m2l_mixed_supported(p->P) : fmmdev::m2l_mfma_supported(p->P))) return FMM_OK;
   try {
-    if (pl->groups_kind != p->precision) {
+    if (pl->groups_kind != kind) {
       pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, mixed,
                                          pl->images > 0 ? pl->cycle : nullptr);
-      pl->groups_kind = p->precision;
+      pl->groups_kind = kind;
       pl->tiles = fmm::M2LTiles();
     }
     const int TT = fmmdev::m2l_tile_targets(p->P);
@@ -710,7 +718,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
       }
     UP(xl_child8, child8) UP(xl_children, children) UP(xl_tiles, tiles) UP(xl_dx, dxs) UP(xl_kinds, kinds)
   }
-  c->mixed = c->p.precision == FMM_MIXED;
+  c->mixed = c->p.precision == FMM_MIXED || c->p.precision == FMM_MIXED16;
+  c->h16 = c->p.precision == FMM_MIXED16;
   c->use_mfma = c->p.m2l_mode != FMM_M2L_TARGET && L.nm2l > 0 &&
                 (c->mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P));
   c->use_tiles = c->use_mfma && (c->p.m2l_mode == FMM_M2L_TILES ||
@@ -755,7 +764,8 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->az, sizeof(double) * n))) return rc;
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
-  if (c->mixed && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
+  if (c->mixed && !c->h16 && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
+  if (c->h16 && (rc = ensure(c, c->M16, sizeof(uint2) * (NT + 1) * nc))) return rc;
   c->cyc = c->p.images > 0 ? make_double3(c->p.cycle[0], c->p.cycle[1], c->p.cycle[2]) : make_double3(0, 0, 0);
   c->far = c->p.images >= 2 && nc > 0;
   if (c->far) {  // traversePeriodic (include/traversal.h:233-287) as one operator on the root
@@ -823,9 +833,9 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     const size_t ny = (size_t)fmmdev::group_harmonics_size(c->p.P) * nr;
     if ((rc = ensure(c, c->g_y, sizeof(double2) * ny))) return rc;
     HIPCHK(fmmdev::launch_group_harmonics(c->p.P, c->c_dx.as<double>(), nr, c->g_y.as<double2>(), c->st));
-    if ((rc = ensure(c, c->g_ops, fmmdev::group_ops_bytes(c->p.P, c->mixed) * nr))) return rc;
-    HIPCHK(fmmdev::launch_group_ops(c->p.P, c->mixed, c->g_y.as<double2>(), c->c_sc.as<double2>(), nr, c->g_ops.p,
-                                    c->st));
+    if ((rc = ensure(c, c->g_ops, fmmdev::group_ops_bytes(c->p.P, c->mixed, c->h16) * nr))) return rc;
+    HIPCHK(fmmdev::launch_group_ops(c->p.P, c->mixed, c->h16, c->g_y.as<double2>(), c->c_sc.as<double2>(), nr,
+                                    c->g_ops.p, c->st));
   }
   double t4 = now_ms();
   pl.tree = fmm::Tree();
@@ -988,14 +998,18 @@ int fmm_m2l(fmm_ctx *c) {
   const double3 xper = c->cyc;  // image shifts of the source codes
   if (c->timing) HIPCHK(hipEventRecord(c->kev[0], c->st));
   if (c->use_mfma) {
-    if (c->mixed)
+    if (c->h16)
+      HIPCHK(fmmdev::launch_m_to_f16(c->p.P, c->M.as<double2>(), c->cr.as<double>(), c->tree.ncell, c->M16.p,
+                                     c->st));
+    else if (c->mixed)
       HIPCHK(fmmdev::launch_m_to_f32(c->p.P, c->M.as<double2>(), c->cr.as<double>(), c->tree.ncell,
                                      c->M32.as<float2>(), c->st));
     if (c->use_tiles) {
-      HIPCHK(fmmdev::launch_m2l_tile(c->p.P, c->mixed, c->ntile, c->t_tgt.as<int>(), c->t_ptr.as<int4>(),
+      HIPCHK(fmmdev::launch_m2l_tile(c->p.P, c->mixed, c->h16, c->ntile, c->t_tgt.as<int>(), c->t_ptr.as<int4>(),
                                      c->t_slice.as<int4>(), c->t_src.as<int>(), c->t_lt.as<unsigned short>(),
-                                     c->g_ops.p, c->cr.as<double>(), c->mixed ? c->M32.p : c->M.p,
-                                     c->L.as<double2>(), c->st));
+                                     c->g_ops.p, c->cr.as<double>(),
+                                     c->h16 ? c->M16.p : (c->mixed ? c->M32.p : c->M.p), c->L.as<double2>(),
+                                     c->st));
     } else if (c->mixed) {
       HIPCHK(fmmdev::launch_m2l_mfma32(c->p.P, c->g_chunk.as<int4>(), c->nchunk, c->g_dx.as<double>(),
                                        c->g_sc.as<double2>(), c->g_tgt.as<int>(), c->g_src.as<int>(),

@@ -62,14 +62,16 @@ hipError_t launch_l2p(int P, DevBodies B, DevCells C, const int *body_leaf, int 
 // a split tile adds into L (which must start at zero).
 int m2l_tile_targets(int P);
 bool m2l_tiles_preferred(int P, bool mixed);  // AUTO uses tiles (else chunks) at this order
-hipError_t launch_m2l_tile(int P, bool mix, int nitem, const int *ttgt, const int4 *items, const int4 *slices,
-                           const int *psrc, const unsigned short *plt, const void *ops, const double *cr,
-                           const void *M, double2 *L, hipStream_t s);
+// h16 (with mix): split-FP16 operands (tables of group_ops_bytes(P, true, true)
+// per group, multipoles from launch_m_to_f16)
+hipError_t launch_m2l_tile(int P, bool mix, bool h16, int nitem, const int *ttgt, const int4 *items,
+                           const int4 *slices, const int *psrc, const unsigned short *plt, const void *ops,
+                           const double *cr, const void *M, double2 *L, hipStream_t s);
 // every group's dense operator as the tile kernel's MFMA A operands (from the
 // group harmonics); group_ops_bytes: bytes per group
-size_t group_ops_bytes(int P, bool mix);
-hipError_t launch_group_ops(int P, bool mix, const double2 *gY, const double2 *gscale, int ngroup, void *ops,
-                            hipStream_t s);
+size_t group_ops_bytes(int P, bool mix, bool h16 = false);
+hipError_t launch_group_ops(int P, bool mix, bool h16, const double2 *gY, const double2 *gscale, int ngroup,
+                            void *ops, hipStream_t s);
 // singular harmonics of every group offset (group_harmonics_size(P) complex each)
 int group_harmonics_size(int P);
 hipError_t launch_group_harmonics(int P, const double *gdx, int ngroup, double2 *Y, hipStream_t s);
@@ -101,6 +103,9 @@ hipError_t launch_l2l_mfma(int P, const int4 *tiles, int ntile, const int *child
 // M2L with per-group scales (R_s, R_t) accumulating into the FP64 L; FP32 P2P
 bool m2l_mixed_supported(int P);
 hipError_t launch_m_to_f32(int P, const double2 *M, const double *R, int ncell, float2 *M32, hipStream_t s);
+// split-FP16 multipoles: per cell NT terms (re hi|lo, im hi|lo halves) and its
+// scale, (nterm(P) + 1) x 8 bytes per cell
+hipError_t launch_m_to_f16(int P, const double2 *M, const double *R, int ncell, void *M16, hipStream_t s);
 hipError_t launch_m2l_mfma32(int P, const int4 *chunks, int nchunk, const double *gdx, const double2 *gscale,
                              const int *ptgt, const int *psrc, const float2 *M32, double2 *L, hipStream_t s);
 // targets[0, nwide) are leaves of at least kP2PWideLeaf bodies (run with four

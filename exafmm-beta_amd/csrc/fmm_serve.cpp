@@ -77,7 +77,7 @@ namespace {
 int64_t footprint(const fmm_params &p, const fmm_stats &s) {
   const int64_t nt = (int64_t)p.P * (p.P + 1) / 2;
   int64_t b = (int64_t)s.n * (8 * 8 + 4);
-  b += (int64_t)s.ncell * (2 * nt * 16 + (p.precision == FMM_MIXED ? nt * 8 + 8 : 0) + 80);
+  b += (int64_t)s.ncell * (2 * nt * 16 + (p.precision != FMM_FP64 ? nt * 8 + 16 : 0) + 80);
   b += s.m2l_pairs * (4 + 8) + s.p2p_pairs * 4 + s.m2l_pairs / 32 * 16;
   return b;
 }

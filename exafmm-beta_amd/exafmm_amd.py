@@ -35,6 +35,7 @@ SYMBOLS = [
 
 FMM_FP64 = 0
 FMM_MIXED = 1
+FMM_MIXED16 = 2  # mixed with the M2L on split-FP16 operands (hi + lo halves, FP32 accumulate)
 M2L_AUTO = 0      # offset groups on MFMA in target tiles (P <= 12), else per target
 M2L_TARGET = 1    # per-target kernel on vector ALUs, fixed summation order
 M2L_GROUPED = 2   # offset groups on MFMA, one block per group chunk, FP64 atomics

@@ -36,8 +36,14 @@ typedef enum {
  * added into FP64 accumulators; M2L as grouped FP32 MFMA translations whose
  * results are added into FP64 local expansions); P2M, M2M, L2L, L2P, the
  * harmonics of every translation and all accumulation across lists stay FP64.
- * Needs P <= 12 and m2l_mode AUTO or GROUPED. */
-typedef enum { FMM_FP64 = 0, FMM_MIXED = 1 } fmm_precision;
+ * Needs P <= 12 and m2l_mode AUTO or GROUPED.
+ * MIXED16: MIXED with the M2L on split-FP16 operands: every operator row
+ * (scaled by a power of two) and every cell's multipole (likewise) are held
+ * as hi + lo halves, applied with v_mfma_f32_16x16x32_f16 as hi x hi + hi x lo
+ * + lo x hi into FP32 accumulators (22 significant bits of every value within
+ * 2^-14 of its row's or cell's largest); accumulation across lists in FP64
+ * as for MIXED.  Needs P <= 12 and m2l_mode AUTO or TILES. */
+typedef enum { FMM_FP64 = 0, FMM_MIXED = 1, FMM_MIXED16 = 2 } fmm_precision;
 
 /* M2L execution.  Pairs are grouped by offset (one dense operator per group,
  * applied on MFMA) in two layouts.  Target tiles: a block owns a tile of
