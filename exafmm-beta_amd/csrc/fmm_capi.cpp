@@ -765,7 +765,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   if ((rc = ensure(c, c->M, sizeof(double2) * NT * nc))) return rc;
   if ((rc = ensure(c, c->L, sizeof(double2) * NT * nc))) return rc;
   if (c->mixed && !c->h16 && (rc = ensure(c, c->M32, sizeof(float2) * NT * nc))) return rc;
-  if (c->h16 && (rc = ensure(c, c->M16, sizeof(uint2) * (NT + 1) * nc))) return rc;
+  if (c->h16 && (rc = ensure(c, c->M16, fmmdev::m16_record_bytes(c->p.P) * nc))) return rc;
   c->cyc = c->p.images > 0 ? make_double3(c->p.cycle[0], c->p.cycle[1], c->p.cycle[2]) : make_double3(0, 0, 0);
   c->far = c->p.images >= 2 && nc > 0;
   if (c->far) {  // traversePeriodic (include/traversal.h:233-287) as one operator on the root

@@ -103,8 +103,9 @@ hipError_t launch_l2l_mfma(int P, const int4 *tiles, int ntile, const int *child
 // M2L with per-group scales (R_s, R_t) accumulating into the FP64 L; FP32 P2P
 bool m2l_mixed_supported(int P);
 hipError_t launch_m_to_f32(int P, const double2 *M, const double *R, int ncell, float2 *M32, hipStream_t s);
-// split-FP16 multipoles: per cell NT terms (re hi|lo, im hi|lo halves) and its
-// scale, (nterm(P) + 1) x 8 bytes per cell
+// split-FP16 multipoles: per cell NT terms (re hi|lo, im hi|lo halves) and
+// its k-block scales, m16_record_bytes(P) per cell
+size_t m16_record_bytes(int P);
 hipError_t launch_m_to_f16(int P, const double2 *M, const double *R, int ncell, void *M16, hipStream_t s);
 hipError_t launch_m2l_mfma32(int P, const int4 *chunks, int nchunk, const double *gdx, const double2 *gscale,
                              const int *ptgt, const int *psrc, const float2 *M32, double2 *L, hipStream_t s);

@@ -48,8 +48,16 @@ def test_mixed16_matches_fp64(P, dist, n):
     e64, e16 = O.rel_l2(d, t64[idx]), O.rel_l2(d, t16[idx])
     print("P=%d %s: mixed16-fp64 %.2e %.2e (mixed-fp64 pot %.2e); vs direct fp64 %.2e %.2e mixed16 %.2e %.2e"
           % (P, dist, dp, da, d32, e64[0], e64[1], e16[0], e16[1]))
+    # worst single target, against the FP32-operand path's worst: the split
+    # operands must not leave outliers (a scale per cell instead of per
+    # (cell, k-block) flushed high-degree components of some interactions)
+    rms = float(np.sqrt(np.mean(t64[:, 0] ** 2)))
+    w16 = float(np.max(np.abs(t16[:, 0] - t64[:, 0]))) / rms
+    w32 = float(np.max(np.abs(t32[:, 0] - t64[:, 0]))) / rms
+    print("   worst target potential / rms: mixed16 %.2e, mixed %.2e" % (w16, w32))
     assert dp < 1e-6 and da < 1e-5
     assert e16[0] < e64[0] + 1e-6 and e16[1] < e64[1] + 5e-6
+    assert w16 < 3 * w32 + 1e-6
 
 
 def test_mixed16_periodic_and_refusals():
