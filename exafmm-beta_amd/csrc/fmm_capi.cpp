@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -361,6 +362,48 @@ int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double the
     return FMM_ERR_MEMORY;
   }
   *out = pl;
+  return FMM_OK;
+}
+
+// fmm_plan_build with the lists from the device traversal (fmm_traverse.hip)
+int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, int ncrit, double theta,
+                          int nspawn) {
+  if (!c || !out) return FMM_ERR_ARG;
+  *out = nullptr;
+  if (n < 0 || (n > 0 && !x) || ncrit < 1 || !(theta > 0) || nspawn < 1)
+    FMM_FAIL(FMM_ERR_ARG, "fmm_plan_build_device: bad n, x, ncrit, theta or nspawn");
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  std::unique_ptr<fmm_plan> pl(new (std::nothrow) fmm_plan());
+  if (!pl) return FMM_ERR_MEMORY;
+  try {
+    pl->n = n;
+    double t0 = now_ms();
+    fmm::get_bounds(n, x, pl->xmin, pl->xmax);
+    double t1 = now_ms();
+    pl->tree = fmm::build_tree(n, x, ncrit, pl->xmin, pl->xmax);
+    double t2 = now_ms();
+    const fmm::Tree &t = pl->tree;
+    fmm::Lists L = fmm::empty_lists(t.ncell);
+    const double xper[3] = {0, 0, 0};
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(fmmdev::device_lists(t.ncell, t.X.data(), t.R.data(), t.nchild.data(), t.ichild.data(), t.nbody.data(),
+                                theta, nspawn, xper, 0, 0, L.m2l_ptr, L.m2l_src, L.p2p_ptr, L.p2p_src,
+                                L.p2p_body_pairs, c->st));
+    L.nm2l = (int64_t)L.m2l_src.size();
+    L.np2p = (int64_t)L.p2p_src.size();
+    for (int i = 0; i < t.ncell; i++) {
+      if (L.m2l_ptr[i + 1] > L.m2l_ptr[i]) L.m2l_targets.push_back(i);
+      if (L.p2p_ptr[i + 1] > L.p2p_ptr[i]) L.p2p_targets.push_back(i);
+    }
+    pl->lists = std::move(L);
+    double t3 = now_ms();
+    pl->t_bounds = t1 - t0;
+    pl->t_tree = t2 - t1;
+    pl->t_lists = t3 - t2;
+  } catch (const std::bad_alloc &) {
+    FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while building the plan");
+  }
+  *out = pl.release();
   return FMM_OK;
 }
 

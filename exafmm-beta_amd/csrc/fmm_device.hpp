@@ -11,6 +11,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <vector>
+
 namespace fmmdev {
 
 struct DevBodies {
@@ -143,6 +146,13 @@ hipError_t launch_ewald(EwBodies B, int n, int ksize, int *kv, double2 *S, doubl
 constexpr int kMaxDevices = 64;
 constexpr int kEwaldScratchMaxK = 32;
 hipError_t init_scratch(int device);
+// The interaction lists of a free-space tree built on the device (the dual
+// traversal breadth first, fmm_traverse.hip): CSR lists sorted by (target,
+// source), the P2P body-pair count; the cell arrays are host arrays.
+hipError_t device_lists(int ncell, const double *X, const double *R, const int *nchild, const int *ichild,
+                        const int *nbody, double theta, int nspawn, const double xper[3], int root_i, int root_j,
+                        std::vector<int> &m2l_ptr, std::vector<int> &m2l_src, std::vector<int> &p2p_ptr,
+                        std::vector<int> &p2p_src, int64_t &body_pairs, hipStream_t s);
 hipError_t launch_ewald_scratch(int device, EwBodies B, int n, int ksize, double3 cyc, double alpha, double sigma,
                                 double cutoff, hipStream_t s);
 hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,

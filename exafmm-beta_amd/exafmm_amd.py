@@ -31,6 +31,7 @@ SYMBOLS = [
     "fmm_dipole_correction", "fmm_md_evaluate", "fmm_plan_list_counts", "fmm_get_list_counts", "fmm_ewald",
     "fmm_plan_build_tree", "fmm_plan_pair", "fmm_kernel_p2p", "fmm_kernel_p2m", "fmm_kernel_m2m",
     "fmm_kernel_m2l", "fmm_kernel_l2l", "fmm_kernel_l2p", "fmm_ewald_noalloc", "fmm_evaluate_graph",
+    "fmm_plan_build_device",
 ]
 
 FMM_FP64 = 0
@@ -129,6 +130,8 @@ def load():
         L.fmm_set_bodies_plan.argtypes = [vp, vp, _dp, _dp]
         L.fmm_plan_build.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_int]
+        L.fmm_plan_build_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_int]
         L.fmm_plan_destroy.argtypes = [vp]
         L.fmm_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.fmm_plan_tree.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
@@ -457,15 +460,22 @@ class FMM:
 
 
 class Plan:
-    """Host-only tree + interaction lists (fmm_plan_*): no device needed."""
+    """Tree + interaction lists (fmm_plan_*), built on the host, or with the
+    lists built on the device of an FMM context (lists_on=)."""
 
-    def __init__(self, x, ncrit=64, theta=0.4, nspawn=5000, tree=None):
+    def __init__(self, x, ncrit=64, theta=0.4, nspawn=5000, tree=None, lists_on=None):
         """Build the tree of x, or, with tree= (dict of iparent, ichild, nchild,
-        ibody, nbody, X, R; x already in tree order), adopt that tree."""
+        ibody, nbody, X, R; x already in tree order), adopt that tree.
+        lists_on=FMM: the lists from that context's device (fmm_plan_build_device)."""
         L = load()
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         self.x = x
         self._h = ctypes.c_void_p()
+        if lists_on is not None:
+            rc = L.fmm_plan_build_device(lists_on._h, ctypes.byref(self._h), len(x), _ptr(x), ncrit, theta,
+                                         nspawn)
+            lists_on._check(rc, "fmm_plan_build_device")
+            return
         if tree is None:
             rc = L.fmm_plan_build(ctypes.byref(self._h), len(x), _ptr(x), ncrit, theta, nspawn)
             if rc != 0:

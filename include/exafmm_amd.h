@@ -110,6 +110,14 @@ int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q);
  * internally; a plan can also be built ahead (e.g. on another host thread). */
 typedef struct fmm_plan fmm_plan;
 int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn);
+/* fmm_plan_build with the pair lists of Traversal::traverse (include/traversal.h:90-177,
+ * 326-352; free space) built on the device of context c: the dual traversal
+ * breadth first, each round testing every frontier pair with the same
+ * acceptance test and split rules.  The same pairs as fmm_plan_build; each
+ * cell's lists are ordered by source cell (the host lists keep the walk's
+ * visiting order).  The bounds and the tree are built on the host. */
+int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, int ncrit, double theta,
+                          int nspawn);
 void fmm_plan_destroy(fmm_plan *pl);
 /* Traversal::countList / writeList (include/traversal.h:30-38, 412-421):
  * per cell (ncell entries each) its Morton key ICELL (BuildTree::getKey,

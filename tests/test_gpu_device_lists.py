@@ -1,0 +1,91 @@
+"""Interaction lists built on the device (fmm_plan_build_device) against the
+host walker (fmm_plan_build, itself equal to the oracle's traversal).
+
+The device traversal runs the dual walk breadth first, so a target's list
+comes out ordered by source cell while the host list keeps the visiting
+order: the lists must hold the same pairs, i.e. every target's sorted host
+list equals its device list, with equal pair and body-pair counts; and a
+solve on either plan gives the same potentials and accelerations (to the
+rounding that the summation order leaves, 1e-12).  Cases: uniform cube and
+Plummer sphere, theta 0.3-0.6, ncrit 8-128, nspawn 100 and 5000 (the
+both-sides split rule), coincident bodies, one body, no bodies, and the
+2^20-body Plummer and 2^21-body cube at the benchmark parameters.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _F():
+    import exafmm_amd
+    return exafmm_amd
+
+
+def _same_lists(hp, dp):
+    hs, ds = hp.stats(), dp.stats()
+    for k in ("ncell", "m2l_pairs", "p2p_pairs", "p2p_body_pairs"):
+        assert hs[k] == ds[k], (k, hs[k], ds[k])
+    hm_ptr, hm_src, hp_ptr, hp_src = hp.lists()
+    dm_ptr, dm_src, dp_ptr, dp_src = dp.lists()
+    assert np.array_equal(hm_ptr, dm_ptr) and np.array_equal(hp_ptr, dp_ptr)
+    for ptr, hsrc, dsrc in ((hm_ptr, hm_src, dm_src), (hp_ptr, hp_src, dp_src)):
+        # every target's sources, sorted: sort by (target, source) in one go
+        tgt = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
+        order = np.lexsort((hsrc, tgt))
+        assert np.array_equal(hsrc[order], dsrc)
+        # the device list is sorted within each target
+        assert np.all((np.diff(dsrc) > 0) | (np.diff(tgt) != 0))
+    return hs
+
+
+@pytest.mark.parametrize("dist,n,theta,ncrit,nspawn", [
+    ("cube1", 20000, 0.4, 64, 5000),
+    ("cube1", 20000, 0.6, 16, 100),
+    ("plummer1", 30000, 0.4, 32, 5000),
+    ("plummer1", 30000, 0.3, 8, 1000),
+    ("cube1", 5000, 0.5, 128, 5000),
+])
+def test_device_lists_equal_host_lists(dist, n, theta, ncrit, nspawn):
+    F = _F()
+    x, q = F.make_bodies(n, dist, 3)
+    f = F.FMM(P=6, theta=theta, ncrit=ncrit, nspawn=nspawn)
+    hp = F.Plan(x, ncrit=ncrit, theta=theta, nspawn=nspawn)
+    dp = F.Plan(x, ncrit=ncrit, theta=theta, nspawn=nspawn, lists_on=f)
+    s = _same_lists(hp, dp)
+    assert s["m2l_pairs"] > 0 and s["p2p_pairs"] > 0
+    f.set_bodies_plan(hp, q)
+    f.evaluate()
+    th = f.targets()
+    f.set_bodies_plan(dp, q)
+    f.evaluate()
+    td = f.targets()
+    f.close()
+    rel = np.linalg.norm(td - th, axis=0) / np.linalg.norm(th, axis=0)
+    assert np.all(rel < 1e-12), rel
+
+
+def test_device_lists_edge_cases():
+    F = _F()
+    f = F.FMM(P=4)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, (3000, 3))
+    x[:300] = x[0]  # 300 coincident bodies: one leaf at the depth cap
+    for xs in (x, x[:1], np.zeros((0, 3))):
+        hp = F.Plan(xs, ncrit=64)
+        dp = F.Plan(xs, ncrit=64, lists_on=f)
+        _same_lists(hp, dp)
+    f.close()
+
+
+@pytest.mark.parametrize("dist,n", [("plummer1", 1 << 20), ("cube1", 1 << 21)])
+def test_device_lists_full_size(dist, n):
+    F = _F()
+    x, q = F.make_bodies(n, dist, 0)
+    f = F.FMM(P=4)
+    hp = F.Plan(x, ncrit=64, theta=0.4, nspawn=5000)
+    dp = F.Plan(x, ncrit=64, theta=0.4, nspawn=5000, lists_on=f)
+    s = _same_lists(hp, dp)
+    print("%s n=%d: %d M2L, %d P2P pairs; lists host %.1f ms, device %.1f ms"
+          % (dist, n, s["m2l_pairs"], s["p2p_pairs"], hp.stats()["host_lists_ms"], dp.stats()["host_lists_ms"]))
+    f.close()
