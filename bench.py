@@ -470,8 +470,8 @@ def main():
                     help="skip the smaller-theta solve that reaches rel-L2 <= 1e-6")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the 2048-request serving streams at 200 and 1000 req/s")
-    ap.add_argument("--no-device-lists", action="store_true",
-                    help="skip the plan whose lists are built on the device (fmm_plan_build_device)")
+    ap.add_argument("--no-device-plan", action="store_true",
+                    help="skip the plan whose tree and lists are built on the device (fmm_plan_build_device)")
     ap.add_argument("--no-graph-small", action="store_true",
                     help="skip the eager-vs-graph latency of serving-sized systems")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
@@ -517,19 +517,23 @@ def main():
     log("[bench] %s: ncell=%d depth=%d m2l=%d p2p=%d body_pairs=%d; host plan %.2f s"
         % (args.workload, st0["ncell"], st0["depth"], st0["m2l_pairs"], st0["p2p_pairs"],
            st0["p2p_body_pairs"], t_plan))
-    dlists = None
-    if not args.no_device_lists:
-        # the same plan with the lists built on the device: the same pairs
-        # (tests/test_gpu_device_lists.py), each target's list in source order
+    dplan_ms = None
+    if not args.no_device_plan:
+        # the same plan with the tree and the lists built on the device: the
+        # same tree and pairs (tests/test_gpu_device_plan.py), each target's
+        # list in source order
         fd = F.FMM(P=4)
         dplan = F.Plan(x, ncrit=w["ncrit"], theta=w["theta"], nspawn=w["nspawn"], lists_on=fd)
         sd = dplan.stats()
         del dplan
         fd.close()
-        dlists = {"lists_ms": sd["host_lists_ms"], "host_lists_ms": st0["host_lists_ms"],
-                  "same_counts": all(sd[k] == st0[k] for k in ("m2l_pairs", "p2p_pairs", "p2p_body_pairs"))}
-        log("[bench] lists on the device %.1f ms (host %.1f ms), same counts %s"
-            % (sd["host_lists_ms"], st0["host_lists_ms"], dlists["same_counts"]))
+        dplan_ms = {"tree_ms": sd["host_tree_ms"], "lists_ms": sd["host_lists_ms"],
+                    "host_tree_ms": st0["host_tree_ms"], "host_lists_ms": st0["host_lists_ms"],
+                    "same_counts": all(sd[k] == st0[k] for k in ("ncell", "m2l_pairs", "p2p_pairs",
+                                                                 "p2p_body_pairs"))}
+        log("[bench] plan on the device: tree %.1f ms (host %.1f), lists %.1f ms (host %.1f), same counts %s"
+            % (sd["host_tree_ms"], st0["host_tree_ms"], sd["host_lists_ms"], st0["host_lists_ms"],
+               dplan_ms["same_counts"]))
     r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log, args.m2l_mode)
     st = r["st"]
     kernels = r["kernels"]
@@ -571,9 +575,10 @@ def main():
         "host_ms": {"bounds": st["host_bounds_ms"], "tree": st["host_tree_ms"],
                     "lists": st["host_lists_ms"], "upload": st["host_upload_ms"]},
         "end_to_end_particles_per_s": n / (1e-3 * (host_ms + float(np.mean(r["dev"])))),
-        "device_lists": None if dlists is None else dict(
-            dlists, end_to_end_particles_per_s=n / (1e-3 * (host_ms - st["host_lists_ms"] + dlists["lists_ms"]
-                                                            + float(np.mean(r["dev"]))))),
+        "device_plan": None if dplan_ms is None else dict(
+            dplan_ms, end_to_end_particles_per_s=n / (1e-3 * (
+                host_ms - st["host_tree_ms"] - st["host_lists_ms"] + dplan_ms["tree_ms"] + dplan_ms["lists_ms"]
+                + float(np.mean(r["dev"]))))),
         "accuracy": r["accuracy"],
         "counts": {"cells": st["ncell"], "leaves": st["nleaf"], "depth": st["depth"],
                    "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"],

@@ -380,12 +380,26 @@ int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, in
     double t0 = now_ms();
     fmm::get_bounds(n, x, pl->xmin, pl->xmax);
     double t1 = now_ms();
-    pl->tree = fmm::build_tree(n, x, ncrit, pl->xmin, pl->xmax);
+    HIPCHK(hipSetDevice(c->p.device));
+    {
+      fmm::Tree &t = pl->tree;
+      t = fmm::Tree();
+      t.n = n;
+      if (n > 0) {
+        double X0[3], R0;
+        fmm::root_box(pl->xmin, pl->xmax, X0, R0);
+        HIPCHK(fmmdev::device_tree(n, x, ncrit, X0, R0, t.perm, t.iparent, t.ichild, t.nchild, t.ibody, t.nbody,
+                                   t.level, t.X, t.R, c->st));
+        t.ncell = (int)t.nbody.size();
+        fmm::finish_tree(t);
+      } else {
+        t.level_ptr.assign(1, 0);
+      }
+    }
     double t2 = now_ms();
     const fmm::Tree &t = pl->tree;
     fmm::Lists L = fmm::empty_lists(t.ncell);
     const double xper[3] = {0, 0, 0};
-    HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(fmmdev::device_lists(t.ncell, t.X.data(), t.R.data(), t.nchild.data(), t.ichild.data(), t.nbody.data(),
                                 theta, nspawn, xper, 0, 0, L.m2l_ptr, L.m2l_src, L.p2p_ptr, L.p2p_src,
                                 L.p2p_body_pairs, c->st));

@@ -146,6 +146,13 @@ hipError_t launch_ewald(EwBodies B, int n, int ksize, int *kv, double2 *S, doubl
 constexpr int kMaxDevices = 64;
 constexpr int kEwaldScratchMaxK = 32;
 hipError_t init_scratch(int device);
+// The octree of n bodies (host array x, AoS) built on the device: the same
+// cells, numbering and body order as fmm::build_tree for the root box (X0,
+// R0) (fmm_traverse.hip); outputs are host arrays.
+hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], double R0, std::vector<int> &perm,
+                       std::vector<int> &iparent, std::vector<int> &ichild, std::vector<int> &nchild,
+                       std::vector<int> &ibody, std::vector<int> &nbody, std::vector<int> &level,
+                       std::vector<double> &X, std::vector<double> &R, hipStream_t s);
 // The interaction lists of a free-space tree built on the device (the dual
 // traversal breadth first, fmm_traverse.hip): CSR lists sorted by (target,
 // source), the P2P body-pair count; the cell arrays are host arrays.

@@ -196,6 +196,32 @@ void nodes2cells(const Builder &B, Tree &t, int node, int C, int CN, double R0, 
 
 }  // namespace
 
+// bounds2box (include/build_tree.h:168-180)
+void root_box(const double xmin[3], const double xmax[3], double X[3], double &R) {
+  R = 0;
+  for (int d = 0; d < 3; d++) X[d] = (xmax[d] + xmin[d]) / 2;
+  for (int d = 0; d < 3; d++) {
+    R = std::max(X[d] - xmin[d], R);
+    R = std::max(xmax[d] - X[d], R);
+  }
+  R *= 1.00001;
+}
+
+void finish_tree(Tree &t) {
+  const int nc = t.ncell;
+  t.depth = 0;
+  for (int c = 0; c < nc; c++) t.depth = std::max(t.depth, t.level[c]);
+  t.level_ptr.assign(t.depth + 2, 0);
+  for (int c = 0; c < nc; c++) t.level_ptr[t.level[c] + 1]++;
+  for (int l = 0; l <= t.depth; l++) t.level_ptr[l + 1] += t.level_ptr[l];
+  t.level_cells.resize(nc);
+  std::vector<int> fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
+  for (int c = 0; c < nc; c++) t.level_cells[fill[t.level[c]]++] = c;
+  t.leaves.clear();
+  for (int c = 0; c < nc; c++)
+    if (t.nchild[c] == 0) t.leaves.push_back(c);
+}
+
 Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const double xmax[3]) {
   Tree t;
   t.n = n;
@@ -204,14 +230,8 @@ Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const d
     t.level_ptr.assign(1, 0);
     return t;
   }
-  // bounds2box (include/build_tree.h:168-180)
-  double X[3], R = 0;
-  for (int d = 0; d < 3; d++) X[d] = (xmax[d] + xmin[d]) / 2;
-  for (int d = 0; d < 3; d++) {
-    R = std::max(X[d] - xmin[d], R);
-    R = std::max(xmax[d] - X[d], R);
-  }
-  R *= 1.00001;
+  double X[3], R;
+  root_box(xmin, xmax, X, R);
   std::vector<int> buffer(n);
   for (int i = 0; i < n; i++) t.perm[i] = i;
   Builder B{x, ncrit, {}};
@@ -271,15 +291,7 @@ Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const d
   t.X.resize(3 * (size_t)nc);
   t.R.resize(nc);
   nodes2cells(B, t, root, 0, 1, R, 0, 0);
-  // level index and leaf list
-  t.level_ptr.assign(t.depth + 2, 0);
-  for (int c = 0; c < nc; c++) t.level_ptr[t.level[c] + 1]++;
-  for (int l = 0; l <= t.depth; l++) t.level_ptr[l + 1] += t.level_ptr[l];
-  t.level_cells.resize(nc);
-  std::vector<int> fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
-  for (int c = 0; c < nc; c++) t.level_cells[fill[t.level[c]]++] = c;
-  for (int c = 0; c < nc; c++)
-    if (t.nchild[c] == 0) t.leaves.push_back(c);
+  finish_tree(t);  // level index and leaf list
   return t;
 }
 

@@ -66,6 +66,10 @@ void get_bounds(int n, const double *x, double xmin[3], double xmax[3]);
 // BuildTree::buildTree (include/build_tree.h:168-209).  x is (n,3) in input
 // order.  Returns the tree; perm gives the body order.
 Tree build_tree(int n, const double *x, int ncrit, const double xmin[3], const double xmax[3]);
+// the root cell of the bounds (bounds2box, include/build_tree.h:168-180)
+void root_box(const double xmin[3], const double xmax[3], double X[3], double &R);
+// depth, level index and leaf list of a tree whose cell arrays are filled
+void finish_tree(Tree &t);
 
 // Wrap a caller-built tree (reference cell order) in a Tree: checks its
 // structure and fills levels, level index and leaves.  Returns an empty

@@ -1,4 +1,4 @@
-// fmm_traverse.hip -- the interaction lists built on the device.
+// fmm_traverse.hip -- the octree and the interaction lists built on the device.
 //
 // The dual tree traversal of include/traversal.h:117-177 (dualTreeTraversal)
 // and :90-114 (splitCell) run breadth first: every round takes the frontier
@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstdint>
 #include <utility>
 #include <vector>
@@ -231,6 +232,206 @@ hipError_t pairs_to_csr(const int2 *pr, size_t n, int ncell, std::vector<int> &p
   return hipSuccess;
 }
 
+
+// ---- the octree (include/build_tree.h:77-209, as fmm_host.cpp builds it) ----
+//
+// Each body's octant path to level 30 is computed with the host builder's
+// centre arithmetic (octant o = (x > X) + 2 (y > Y) + 4 (z > Z), child centre
+// X + r (+-1), r = R0 / 2^(level + 1)), and the bodies are sorted stably by
+// it.  A node of level d is then a run of bodies sharing d octants; a body's
+// leaf is its first run of at most ncrit bodies (or its level-30 run).  The
+// runs come from lcp[j], the octants shared by sorted bodies j - 1 and j,
+// with a sparse table of its minima: the run of level d around j ends at the
+// first k > j with lcp[k] < d and starts at the last k <= j with lcp[k] < d.
+// Cells are enumerated in pre-order (position, level); the cell numbering of
+// nodes2cells (children contiguous, each child's descendants after all
+// children, in child order) follows top-down from the subtree sizes.  Bodies
+// keep their input order within a leaf, as the host's stable partitions do.
+
+__global__ void tree_keys_kernel(int n, const double *__restrict__ x, double X0, double X1, double X2, double R0,
+                                 unsigned long long *__restrict__ hi, unsigned long long *__restrict__ lo,
+                                 int *__restrict__ idx) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double b0 = x[3 * (size_t)i], b1 = x[3 * (size_t)i + 1], b2 = x[3 * (size_t)i + 2];
+  double c0 = X0, c1 = X1, c2 = X2;
+  unsigned long long h = 0, w = 0;
+  for (int lev = 0; lev < 30; lev++) {
+    const int o = (b0 > c0) + ((b1 > c1) << 1) + ((b2 > c2) << 2);
+    if (lev < 21)
+      h |= (unsigned long long)o << (3 * (20 - lev));
+    else
+      w |= (unsigned long long)o << (3 * (29 - lev));
+    const double r = R0 / (1 << (lev + 1));
+    c0 += r * ((o & 1) * 2 - 1);
+    c1 += r * (((o >> 1) & 1) * 2 - 1);
+    c2 += r * (((o >> 2) & 1) * 2 - 1);
+  }
+  hi[i] = h;
+  lo[i] = w;
+  idx[i] = i;
+}
+
+__global__ void gather_u64_kernel(int n, const unsigned long long *__restrict__ src, const int *__restrict__ idx,
+                                  unsigned long long *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// lcp[j] (j = 0..n): octants shared by sorted bodies j - 1 and j; -1 at both ends
+__global__ void tree_lcp_kernel(int n, const unsigned long long *__restrict__ hi,
+                                const unsigned long long *__restrict__ lo, signed char *__restrict__ lcp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  int v = -1;
+  if (j > 0 && j < n) {
+    const unsigned long long a = hi[j] ^ hi[j - 1];
+    if (a) {
+      v = (__clzll(a) - 1) / 3;
+    } else {
+      const unsigned long long b = lo[j] ^ lo[j - 1];
+      v = b ? 21 + (__clzll(b) - 37) / 3 : 30;
+    }
+  }
+  lcp[j] = (signed char)v;
+}
+
+// table level lv: minimum of lcp over [p, p + 2^lv) (positions past n count as 127)
+__global__ void tree_table_kernel(int m, const signed char *__restrict__ prev, int half,
+                                  signed char *__restrict__ cur) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const signed char a = prev[p];
+  const signed char b = p + half < m ? prev[p + half] : (signed char)127;
+  cur[p] = a < b ? a : b;
+}
+
+struct Runs {
+  const signed char *tbl;  // (LG + 1) levels of m = n + 1 entries
+  int m, LG;
+  // first k > j with lcp[k] < d (at most n)
+  __device__ int next(int j, int d) const {
+    int pos = j + 1;
+    if (tbl[pos] < d) return pos;
+    for (int lv = LG; lv >= 0; lv--)
+      if (pos + (1 << lv) <= m && tbl[(size_t)lv * m + pos] >= d) pos += 1 << lv;
+    return pos;
+  }
+  // last k <= j with lcp[k] < d (at least 0)
+  __device__ int prev(int j, int d) const {
+    int pos = j;
+    if (tbl[pos] < d) return pos;
+    for (int lv = LG; lv >= 0; lv--)
+      if (pos - (1 << lv) >= 0 && tbl[(size_t)lv * m + pos - (1 << lv)] >= d) pos -= 1 << lv;
+    return pos - 1;
+  }
+};
+
+// leaf level L[j] of every sorted body, the start of its leaf, and the number
+// of cells starting at position j (levels lcp[j] + 1 .. L[j])
+__global__ void tree_leaf_kernel(int n, Runs R, int ncrit, signed char *__restrict__ L, int *__restrict__ lstart,
+                                 int *__restrict__ cnt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  if (j == n) {
+    cnt[j] = 0;
+    return;
+  }
+  int lo = 0, hi = 30;  // the first level whose run holds <= ncrit bodies, else 30
+  while (lo < hi) {
+    const int d = (lo + hi) >> 1;
+    if (R.next(j, d) - R.prev(j, d) <= ncrit)
+      hi = d;
+    else
+      lo = d + 1;
+  }
+  L[j] = (signed char)lo;
+  lstart[j] = R.prev(j, lo);
+  const int c = lo - R.tbl[j];
+  cnt[j] = c > 0 ? c : 0;
+}
+
+struct PreCells {
+  int *pos, *level, *nbody, *nnode, *nchild;
+};
+
+// every cell (pre-order id cstart[j] + d - lcp[j] - 1 for level d at position j)
+__global__ void tree_cells_kernel(int n, Runs R, const signed char *__restrict__ L, const int *__restrict__ cstart,
+                                  PreCells C) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int d0 = R.tbl[j] + 1, d1 = L[j];
+  for (int d = d0; d <= d1; d++) {
+    const int p = cstart[j] + (d - d0);
+    const int e = R.next(j, d);
+    int nc = 0;
+    if (d < d1)
+      for (int k = j; k < e; k = R.next(k, d + 1)) nc++;
+    C.pos[p] = j;
+    C.level[p] = d;
+    C.nbody[p] = e - j;
+    C.nnode[p] = cstart[e] - p;
+    C.nchild[p] = nc;
+  }
+}
+
+// nodes2cells numbering, one level: every cell of level d numbers its children
+__global__ void tree_number_kernel(int ncell, int d, Runs R, const int *__restrict__ cstart, PreCells C,
+                                   int *__restrict__ fidx, int *__restrict__ fbase, int *__restrict__ fpar) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncell || C.level[p] != d || C.nchild[p] == 0) return;
+  const int j = C.pos[p], e = j + C.nbody[p], base = fbase[p], nc = C.nchild[p];
+  int i = 0, off = base + nc;
+  for (int k = j; k < e; k = R.next(k, d + 1), i++) {
+    const int pc = cstart[k] + (d + 1) - (R.tbl[k] + 1);  // the child's pre-order id (level d + 1 at k)
+    fidx[pc] = base + i;
+    fbase[pc] = off;
+    fpar[pc] = fidx[p];
+    off += C.nnode[pc] - 1;
+  }
+}
+
+// the cell arrays in nodes2cells order
+__global__ void tree_scatter_kernel(int ncell, PreCells C, const int *__restrict__ fidx, const int *__restrict__ fbase,
+                                    const int *__restrict__ fpar, const unsigned long long *__restrict__ hi,
+                                    const unsigned long long *__restrict__ lo, double X0, double X1, double X2,
+                                    double R0, int *__restrict__ iparent, int *__restrict__ ichild,
+                                    int *__restrict__ nchild, int *__restrict__ ibody, int *__restrict__ nbody,
+                                    int *__restrict__ level, double *__restrict__ X, double *__restrict__ Rc) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncell) return;
+  const int f = fidx[p], j = C.pos[p], d = C.level[p];
+  iparent[f] = fpar[p];
+  nchild[f] = C.nchild[p];
+  ichild[f] = C.nchild[p] ? fbase[p] : 0;
+  ibody[f] = j;
+  nbody[f] = C.nbody[p];
+  level[f] = d;
+  double c0 = X0, c1 = X1, c2 = X2;
+  const unsigned long long h = hi[j], w = lo[j];
+  for (int lev = 0; lev < d; lev++) {
+    const int o = lev < 21 ? (int)((h >> (3 * (20 - lev))) & 7) : (int)((w >> (3 * (29 - lev))) & 7);
+    const double r = R0 / (1 << (lev + 1));
+    c0 += r * ((o & 1) * 2 - 1);
+    c1 += r * (((o >> 1) & 1) * 2 - 1);
+    c2 += r * (((o >> 2) & 1) * 2 - 1);
+  }
+  X[3 * (size_t)f] = c0;
+  X[3 * (size_t)f + 1] = c1;
+  X[3 * (size_t)f + 2] = c2;
+  Rc[f] = R0 / (1 << d);
+}
+
+// the leaf start of every body, by input index
+__global__ void tree_leafkey_kernel(int n, const int *__restrict__ idx, const int *__restrict__ lstart,
+                                    unsigned *__restrict__ key, int *__restrict__ iota) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  key[idx[j]] = (unsigned)lstart[j];
+  iota[j] = j;
+}
 }  // namespace
 
 hipError_t device_lists(int ncell, const double *X, const double *R, const int *nchild, const int *ichild,
@@ -311,6 +512,131 @@ hipError_t device_lists(int ncell, const double *X, const double *R, const int *
   }
   TCHK(pairs_to_csr(M.as<int2>(), nm, ncell, m2l_ptr, m2l_src, s));
   TCHK(pairs_to_csr(Pp.as<int2>(), np, ncell, p2p_ptr, p2p_src, s));
+  return hipSuccess;
+}
+
+hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], double R0, std::vector<int> &perm,
+                       std::vector<int> &iparent, std::vector<int> &ichild, std::vector<int> &nchild,
+                       std::vector<int> &ibody, std::vector<int> &nbody, std::vector<int> &level,
+                       std::vector<double> &X, std::vector<double> &R, hipStream_t s) {
+  perm.resize(n);
+  if (n == 0) {
+    for (auto *v : {&iparent, &ichild, &nchild, &ibody, &nbody, &level}) v->clear();
+    X.clear();
+    R.clear();
+    return hipSuccess;
+  }
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  using u64 = unsigned long long;
+  Buf dx(s), kh(s), kl(s), ka(s), kb(s), idx(s), idx2(s), tmp(s);
+  TCHK(dx.need(sizeof(double) * 3 * (size_t)n));
+  TCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+  for (Buf *b : {&kh, &kl, &ka, &kb}) TCHK(b->need(8 * (size_t)n));
+  TCHK(idx.need(4 * (size_t)n));
+  TCHK(idx2.need(4 * (size_t)n));
+  tree_keys_kernel<<<nb, 256, 0, s>>>(n, dx.as<double>(), X0[0], X0[1], X0[2], R0, kh.as<u64>(), kl.as<u64>(),
+                                      idx.as<int>());
+  TCHK(hipGetLastError());
+  // stable sort by the path, least significant part first: by levels 21-29
+  // (kl), then by levels 0-20 (kh, gathered into the first sort's order)
+  size_t tb = 0, tb2 = 0;
+  TCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kl.as<u64>(), ka.as<u64>(), idx.as<int>(), idx2.as<int>(), n,
+                                          0, 27, s));
+  TCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, ka.as<u64>(), kb.as<u64>(), idx2.as<int>(), idx.as<int>(), n,
+                                          0, 63, s));
+  TCHK(tmp.need(std::max(tb, tb2)));
+  TCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kl.as<u64>(), ka.as<u64>(), idx.as<int>(), idx2.as<int>(), n,
+                                          0, 27, s));
+  gather_u64_kernel<<<nb, 256, 0, s>>>(n, kh.as<u64>(), idx2.as<int>(), ka.as<u64>());
+  TCHK(hipGetLastError());
+  TCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb2, ka.as<u64>(), kb.as<u64>(), idx2.as<int>(), idx.as<int>(), n,
+                                          0, 63, s));
+  gather_u64_kernel<<<nb, 256, 0, s>>>(n, kl.as<u64>(), idx.as<int>(), ka.as<u64>());
+  TCHK(hipGetLastError());
+  // sorted paths: levels 0-20 in kb, 21-29 in ka; idx: input index of every sorted body
+  const u64 *hiS = kb.as<u64>(), *loS = ka.as<u64>();
+  const int m = n + 1;
+  int LG = 0;
+  while ((1 << (LG + 1)) <= m) LG++;
+  Buf tbl(s), Lv(s), lst(s), cnt(s), cst(s);
+  TCHK(tbl.need((size_t)(LG + 1) * m));
+  tree_lcp_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(n, hiS, loS, tbl.as<signed char>());
+  TCHK(hipGetLastError());
+  for (int lv = 1; lv <= LG; lv++) {
+    tree_table_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(
+        m, tbl.as<signed char>() + (size_t)(lv - 1) * m, 1 << (lv - 1), tbl.as<signed char>() + (size_t)lv * m);
+    TCHK(hipGetLastError());
+  }
+  Runs RR{tbl.as<signed char>(), m, LG};
+  TCHK(Lv.need(m));
+  TCHK(lst.need(4 * (size_t)m));
+  TCHK(cnt.need(4 * (size_t)m));
+  TCHK(cst.need(4 * (size_t)m));
+  tree_leaf_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(n, RR, ncrit, Lv.as<signed char>(), lst.as<int>(),
+                                                               cnt.as<int>());
+  TCHK(hipGetLastError());
+  size_t ts = 0;
+  TCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, ts, cnt.as<int>(), cst.as<int>(), m, s));
+  Buf tmp2(s);
+  TCHK(tmp2.need(ts));
+  TCHK(hipcub::DeviceScan::ExclusiveSum(tmp2.p, ts, cnt.as<int>(), cst.as<int>(), m, s));
+  int ncell = 0;
+  TCHK(hipMemcpyAsync(&ncell, cst.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  TCHK(hipStreamSynchronize(s));
+  Buf cpos(s), clev(s), cnb(s), cnn(s), cnc(s), fidx(s), fbase(s), fpar(s);
+  for (Buf *b : {&cpos, &clev, &cnb, &cnn, &cnc, &fidx, &fbase, &fpar}) TCHK(b->need(4 * (size_t)ncell));
+  PreCells C{cpos.as<int>(), clev.as<int>(), cnb.as<int>(), cnn.as<int>(), cnc.as<int>()};
+  tree_cells_kernel<<<nb, 256, 0, s>>>(n, RR, Lv.as<signed char>(), cst.as<int>(), C);
+  TCHK(hipGetLastError());
+  // the root: index 0, children from 1, parent 0
+  const int root[3] = {0, 1, 0};
+  TCHK(hipMemcpyAsync(fidx.p, &root[0], sizeof(int), hipMemcpyHostToDevice, s));
+  TCHK(hipMemcpyAsync(fbase.p, &root[1], sizeof(int), hipMemcpyHostToDevice, s));
+  TCHK(hipMemcpyAsync(fpar.p, &root[2], sizeof(int), hipMemcpyHostToDevice, s));
+  const unsigned nbc = (unsigned)((ncell + 255) / 256);
+  for (int d = 0; d < 30; d++) {
+    tree_number_kernel<<<nbc, 256, 0, s>>>(ncell, d, RR, cst.as<int>(), C, fidx.as<int>(), fbase.as<int>(),
+                                           fpar.as<int>());
+    TCHK(hipGetLastError());
+  }
+  Buf oip(s), oic(s), onc(s), oib(s), onb(s), olv(s), oX(s), oR(s);
+  for (Buf *b : {&oip, &oic, &onc, &oib, &onb, &olv}) TCHK(b->need(4 * (size_t)ncell));
+  TCHK(oX.need(24 * (size_t)ncell));
+  TCHK(oR.need(8 * (size_t)ncell));
+  tree_scatter_kernel<<<nbc, 256, 0, s>>>(ncell, C, fidx.as<int>(), fbase.as<int>(), fpar.as<int>(),
+                                          hiS, loS, X0[0], X0[1], X0[2], R0, oip.as<int>(), oic.as<int>(), onc.as<int>(), oib.as<int>(),
+                                          onb.as<int>(), olv.as<int>(), oX.as<double>(), oR.as<double>());
+  TCHK(hipGetLastError());
+  // bodies: by leaf, input order within a leaf
+  Buf key(s), key2(s), io(s), io2(s);
+  TCHK(key.need(4 * (size_t)n));
+  TCHK(key2.need(4 * (size_t)n));
+  TCHK(io.need(4 * (size_t)n));
+  TCHK(io2.need(4 * (size_t)n));
+  tree_leafkey_kernel<<<nb, 256, 0, s>>>(n, idx.as<int>(), lst.as<int>(), key.as<unsigned>(), io.as<int>());
+  TCHK(hipGetLastError());
+  int nbits = 1;
+  while ((1ll << nbits) < (long long)n) nbits++;
+  size_t tk = 0;
+  TCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tk, key.as<unsigned>(), key2.as<unsigned>(), io.as<int>(),
+                                          io2.as<int>(), n, 0, nbits, s));
+  Buf tmp3(s);
+  TCHK(tmp3.need(tk));
+  TCHK(hipcub::DeviceRadixSort::SortPairs(tmp3.p, tk, key.as<unsigned>(), key2.as<unsigned>(), io.as<int>(),
+                                          io2.as<int>(), n, 0, nbits, s));
+  for (auto *v : {&iparent, &ichild, &nchild, &ibody, &nbody, &level}) v->resize(ncell);
+  X.resize(3 * (size_t)ncell);
+  R.resize(ncell);
+  TCHK(hipMemcpyAsync(perm.data(), io2.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(iparent.data(), oip.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(ichild.data(), oic.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(nchild.data(), onc.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(ibody.data(), oib.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(nbody.data(), onb.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(level.data(), olv.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(X.data(), oX.p, 24 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(R.data(), oR.p, 8 * (size_t)ncell, hipMemcpyDeviceToHost, s));
+  TCHK(hipStreamSynchronize(s));
   return hipSuccess;
 }
 

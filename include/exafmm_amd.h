@@ -110,12 +110,16 @@ int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q);
  * internally; a plan can also be built ahead (e.g. on another host thread). */
 typedef struct fmm_plan fmm_plan;
 int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double theta, int nspawn);
-/* fmm_plan_build with the pair lists of Traversal::traverse (include/traversal.h:90-177,
- * 326-352; free space) built on the device of context c: the dual traversal
- * breadth first, each round testing every frontier pair with the same
- * acceptance test and split rules.  The same pairs as fmm_plan_build; each
- * cell's lists are ordered by source cell (the host lists keep the walk's
- * visiting order).  The bounds and the tree are built on the host. */
+/* fmm_plan_build with the tree (BuildTree::buildTree, include/build_tree.h:77-209)
+ * and the pair lists of Traversal::traverse (include/traversal.h:90-177,
+ * 326-352; free space) built on the device of context c.  The tree: every
+ * body's octant path from the same centre arithmetic, one stable sort, and
+ * the runs of shared octants as cells -- the same cells, cell numbering and
+ * body order as fmm_plan_build.  The lists: the dual traversal breadth first,
+ * each round testing every frontier pair with the same acceptance test and
+ * split rules -- the same pairs as fmm_plan_build, each cell's lists ordered
+ * by source cell (the host lists keep the walk's visiting order).  The
+ * bounds are computed on the host (their sequential 1e-5 drift). */
 int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, int ncrit, double theta,
                           int nspawn);
 void fmm_plan_destroy(fmm_plan *pl);

@@ -1,15 +1,18 @@
-"""Interaction lists built on the device (fmm_plan_build_device) against the
-host walker (fmm_plan_build, itself equal to the oracle's traversal).
+"""The plan built on the device (fmm_plan_build_device) against the host plan
+(fmm_plan_build, whose tree and lists equal the oracle's).
 
-The device traversal runs the dual walk breadth first, so a target's list
-comes out ordered by source cell while the host list keeps the visiting
+Tree: the same cells, cell numbering, centres, radii and body permutation,
+bit for bit (every body's octant path comes from the same centre arithmetic).
+Lists: the device traversal runs the dual walk breadth first, so a target's
+list comes out ordered by source cell while the host list keeps the visiting
 order: the lists must hold the same pairs, i.e. every target's sorted host
 list equals its device list, with equal pair and body-pair counts; and a
 solve on either plan gives the same potentials and accelerations (to the
 rounding that the summation order leaves, 1e-12).  Cases: uniform cube and
 Plummer sphere, theta 0.3-0.6, ncrit 8-128, nspawn 100 and 5000 (the
-both-sides split rule), coincident bodies, one body, no bodies, and the
-2^20-body Plummer and 2^21-body cube at the benchmark parameters.
+both-sides split rule), more than ncrit coincident bodies (a leaf at the
+depth cap), one body, no bodies, and the 2^20-body Plummer and 2^21-body
+cube at the benchmark parameters.
 """
 import numpy as np
 import pytest
@@ -22,7 +25,15 @@ def _F():
     return exafmm_amd
 
 
+def _same_tree(hp, dp):
+    ht, dt = hp.tree(), dp.tree()
+    assert ht["depth"] == dt["depth"]
+    for k in ("perm", "iparent", "ichild", "nchild", "ibody", "nbody", "X", "R"):
+        assert ht[k].shape == dt[k].shape and np.array_equal(ht[k], dt[k]), k
+
+
 def _same_lists(hp, dp):
+    _same_tree(hp, dp)
     hs, ds = hp.stats(), dp.stats()
     for k in ("ncell", "m2l_pairs", "p2p_pairs", "p2p_body_pairs"):
         assert hs[k] == ds[k], (k, hs[k], ds[k])
@@ -46,7 +57,7 @@ def _same_lists(hp, dp):
     ("plummer1", 30000, 0.3, 8, 1000),
     ("cube1", 5000, 0.5, 128, 5000),
 ])
-def test_device_lists_equal_host_lists(dist, n, theta, ncrit, nspawn):
+def test_device_plan_equals_host_plan(dist, n, theta, ncrit, nspawn):
     F = _F()
     x, q = F.make_bodies(n, dist, 3)
     f = F.FMM(P=6, theta=theta, ncrit=ncrit, nspawn=nspawn)
@@ -65,7 +76,7 @@ def test_device_lists_equal_host_lists(dist, n, theta, ncrit, nspawn):
     assert np.all(rel < 1e-12), rel
 
 
-def test_device_lists_edge_cases():
+def test_device_plan_edge_cases():
     F = _F()
     f = F.FMM(P=4)
     rng = np.random.default_rng(5)
@@ -79,13 +90,14 @@ def test_device_lists_edge_cases():
 
 
 @pytest.mark.parametrize("dist,n", [("plummer1", 1 << 20), ("cube1", 1 << 21)])
-def test_device_lists_full_size(dist, n):
+def test_device_plan_full_size(dist, n):
     F = _F()
     x, q = F.make_bodies(n, dist, 0)
     f = F.FMM(P=4)
     hp = F.Plan(x, ncrit=64, theta=0.4, nspawn=5000)
     dp = F.Plan(x, ncrit=64, theta=0.4, nspawn=5000, lists_on=f)
     s = _same_lists(hp, dp)
-    print("%s n=%d: %d M2L, %d P2P pairs; lists host %.1f ms, device %.1f ms"
-          % (dist, n, s["m2l_pairs"], s["p2p_pairs"], hp.stats()["host_lists_ms"], dp.stats()["host_lists_ms"]))
+    print("%s n=%d: %d M2L, %d P2P pairs; tree host %.1f ms, device %.1f ms; lists host %.1f ms, device %.1f ms"
+          % (dist, n, s["m2l_pairs"], s["p2p_pairs"], hp.stats()["host_tree_ms"], dp.stats()["host_tree_ms"],
+             hp.stats()["host_lists_ms"], dp.stats()["host_lists_ms"]))
     f.close()
