@@ -84,6 +84,16 @@ int64_t footprint(const fmm_params &p, const fmm_stats &s) {
 
 void builder_loop(fmm_server *s) {
   fmm::set_host_threads(1);  // requests are planned in parallel, each on one thread
+  // plan_on_device: this builder's own context (stream) for the tree and lists
+  fmm_ctx *bc = nullptr;
+  if (s->cfg.plan_on_device && fmm_create(&bc, &s->p) != FMM_OK) {
+    fmm_destroy(bc);
+    bc = nullptr;  // plan on the host instead
+  }
+  struct Release {
+    fmm_ctx *c;
+    ~Release() { fmm_destroy(c); }
+  } release{bc};
   for (;;) {
     std::shared_ptr<Request> r;
     {
@@ -94,7 +104,8 @@ void builder_loop(fmm_server *s) {
       s->to_plan.pop_front();
     }
     fmm_plan *pl = nullptr;
-    int rc = fmm_plan_build(&pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn);
+    int rc = bc ? fmm_plan_build_device(bc, &pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn)
+                : fmm_plan_build(&pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn);
     if (rc == FMM_OK) rc = fmm_plan_prepare(pl, &s->p);
     fmm_stats st{};
     if (rc == FMM_OK) fmm_plan_stats(pl, &st);
@@ -257,6 +268,7 @@ void fmm_server_default_config(fmm_server_config *cfg) {
   cfg->batch_bodies = 1 << 20;
   cfg->batch_requests = 64;
   cfg->batch_delay_ms = 0.0;
+  cfg->plan_on_device = 0;
 }
 
 int fmm_server_create(fmm_server **out, const fmm_params *p, const fmm_server_config *cfg) {

@@ -46,7 +46,8 @@ M2L_TILES = 3     # offset groups on MFMA, one block per target tile, LDS accumu
 class ServerConfig(ctypes.Structure):
     _fields_ = [("host_threads", ctypes.c_int), ("slots", ctypes.c_int),
                 ("device_budget", ctypes.c_int64), ("batch_bodies", ctypes.c_int),
-                ("batch_requests", ctypes.c_int), ("batch_delay_ms", ctypes.c_double)]
+                ("batch_requests", ctypes.c_int), ("batch_delay_ms", ctypes.c_double),
+                ("plan_on_device", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -338,6 +338,8 @@ typedef struct {
   int batch_bodies;       /* a batch closes at this many bodies ... */
   int batch_requests;     /* ... or this many requests */
   double batch_delay_ms;  /* hold a batch open up to this long for more requests (0: never) */
+  int plan_on_device;     /* 1: each builder builds the tree and lists on the device
+                           * (fmm_plan_build_device, one context and stream per builder) */
 } fmm_server_config;
 
 typedef struct {

@@ -64,6 +64,19 @@ def test_results_equal_alone_and_in_mix(precision, mode):
         assert _rel(ta[:, 0], t[:, 0]) < MIX_TOL and _rel(ta[:, 1:], t[:, 1:]) < MIX_TOL
 
 
+def test_plans_built_on_the_device():
+    """plan_on_device: the builders build trees and lists on the device (the
+    same trees and pairs, lists in source order): results equal the
+    host-planned loop's to the rounding of the summation order."""
+    F = _F()
+    systems = _systems(F, 12, 2)
+    host, _ = _serve(F, systems, F.FMM_FP64, slots=3, batch_requests=64)
+    dev, st = _serve(F, systems, F.FMM_FP64, slots=3, batch_requests=64, plan_on_device=1)
+    assert st["requests_done"] == 12
+    for (th, _), (td, _) in zip(host, dev):
+        assert _rel(td[:, 0], th[:, 0]) < MIX_TOL and _rel(td[:, 1:], th[:, 1:]) < MIX_TOL
+
+
 def test_memory_admission_serialises_batches():
     """A budget below any footprint admits a batch only when none is in
     flight: batch intervals never overlap."""
