@@ -327,7 +327,8 @@ int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p
  * or batch_requests requests -- once their device footprint fits
  * device_budget beside the batches in flight, and evaluates them as one
  * forest (one launch per phase for the whole batch).  Batches of different
- * slots overlap on the GPU.  A system's result does not depend on its batch
+ * slots overlap on the GPU.  With m2l_mode AUTO the loop's M2L runs as grouped
+ * chunks (FMM_M2L_GROUPED): no tiles to plan, since planning bounds it.  A system's result does not depend on its batch
  * beyond the order of FP64 atomic additions into its local expansions. */
 typedef struct fmm_server fmm_server;
 
