@@ -228,7 +228,10 @@ def cpu_baseline(x, q, w, plan, budget_s):
     pass of the same solve -- P2M and L2P of every s-th leaf, M2M of every
     s-th internal cell, L2L of every s-th cell, and every k-th pair of the
     M2L and P2P lists -- each phase scaled by its share of the whole work.
-    The tree and lists are the host plan's (identical to the oracle's).
+    The tree build and the traversal, which the reference's "Total FMM"
+    timer also covers, are timed whole on one core with the oracle's C
+    restatement (the reference's tree and traversal headers need its
+    generated config.h; the restatement makes the same tree and pairs).
     Falls back to the oracle's C restatement (whole upward/downward passes)
     when _ref is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -258,8 +261,15 @@ def cpu_baseline(x, q, w, plan, budget_s):
                 float(nc - 1), float(n)]
         phase = {k: t[i] * full[i] / max(work[i], 1.0)
                  for i, k in enumerate(("p2m", "m2m", "m2l", "p2p", "l2l", "l2p"))}
+        t0 = time.perf_counter()
+        otree = O.Tree(x, w["ncrit"])
+        t1 = time.perf_counter()
+        otree.traverse(w["theta"], w["nspawn"])
+        t2 = time.perf_counter()
+        del otree
+        phase.update(tree=t1 - t0, lists=t2 - t1)
         total = sum(phase.values())
-        kind, sampled = "reference", float(t.sum())
+        kind, sampled = "reference", float(t.sum()) + (t2 - t0)
     else:
         tree = O.Tree(x, w["ncrit"])
         k = O.Kernel(P)
@@ -283,9 +293,11 @@ def cpu_baseline(x, q, w, plan, budget_s):
         "cores": 1,
         "kind": kind,
         "sample": ("%s: every %d-th cell of each pass (P2M, M2M, L2L, L2P), every %d-th of %d M2L "
-                   "pairs and every %d-th of %d P2P pairs timed on one core (%.1f s) and each phase "
-                   "scaled to the whole solve (est. %.1f s per solve at N=%d)"
-                   % (w["desc"], s_pass, s_m2l, len(mi), s_p2p, len(pi), sampled, total, n)),
+                   "pairs and every %d-th of %d P2P pairs timed on one core and each phase "
+                   "scaled to the whole solve%s (%.1f s timed; est. %.1f s per solve at N=%d)"
+                   % (w["desc"], s_pass, s_m2l, len(mi), s_p2p, len(pi),
+                      "; the tree build and traversal timed whole with the oracle's C restatement"
+                      if "tree" in phase else "", sampled, total, n)),
         "n": n,
         "phase_s": phase,
     }
