@@ -533,21 +533,29 @@ def main():
            st0["p2p_body_pairs"], t_plan))
     dplan_ms = None
     if not args.no_device_plan:
-        # the same plan with the tree and the lists built on the device: the
-        # same tree and pairs (tests/test_gpu_device_plan.py), each target's
-        # list in source order
-        fd = F.FMM(P=4)
+        # the same plan with the tree, the lists and (FP64) the M2L groups and
+        # tiles built on the device: the same tree, pairs and groups
+        # (tests/test_gpu_device_plan.py), each target's list in source
+        # order; then uploaded into an FP64 context, timed as a whole
+        fd = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"])
+        t0 = time.perf_counter()
         dplan = F.Plan(x, ncrit=w["ncrit"], theta=w["theta"], nspawn=w["nspawn"], lists_on=fd)
-        sd = dplan.stats()
+        fd.set_bodies_plan(dplan, q)
+        fd.synchronize()
+        t1 = time.perf_counter()
+        sd, su = dplan.stats(), fd.stats()
         del dplan
         fd.close()
-        dplan_ms = {"tree_ms": sd["host_tree_ms"], "lists_ms": sd["host_lists_ms"],
+        dplan_ms = {"bounds_ms": sd["host_bounds_ms"], "tree_ms": sd["host_tree_ms"],
+                    "lists_ms": sd["host_lists_ms"], "groups_tiles_ms": sd["plan_groups_ms"],
+                    "upload_ms": su["host_upload_ms"], "total_ms": 1e3 * (t1 - t0),
                     "host_tree_ms": st0["host_tree_ms"], "host_lists_ms": st0["host_lists_ms"],
                     "same_counts": all(sd[k] == st0[k] for k in ("ncell", "m2l_pairs", "p2p_pairs",
                                                                  "p2p_body_pairs"))}
-        log("[bench] plan on the device: tree %.1f ms (host %.1f), lists %.1f ms (host %.1f), same counts %s"
+        log("[bench] plan on the device: tree %.1f ms (host %.1f), lists %.1f ms (host %.1f), groups and tiles "
+            "%.1f ms, upload %.1f ms; %.0f ms in all; same counts %s"
             % (sd["host_tree_ms"], st0["host_tree_ms"], sd["host_lists_ms"], st0["host_lists_ms"],
-               dplan_ms["same_counts"]))
+               sd["plan_groups_ms"], su["host_upload_ms"], dplan_ms["total_ms"], dplan_ms["same_counts"]))
     r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log, args.m2l_mode)
     st = r["st"]
     kernels = r["kernels"]
@@ -590,9 +598,7 @@ def main():
                     "lists": st["host_lists_ms"], "upload": st["host_upload_ms"]},
         "end_to_end_particles_per_s": n / (1e-3 * (host_ms + float(np.mean(r["dev"])))),
         "device_plan": None if dplan_ms is None else dict(
-            dplan_ms, end_to_end_particles_per_s=n / (1e-3 * (
-                host_ms - st["host_tree_ms"] - st["host_lists_ms"] + dplan_ms["tree_ms"] + dplan_ms["lists_ms"]
-                + float(np.mean(r["dev"]))))),
+            dplan_ms, end_to_end_particles_per_s=n / (1e-3 * (dplan_ms["total_ms"] + float(np.mean(r["dev"]))))),
         "accuracy": r["accuracy"],
         "counts": {"cells": st["ncell"], "leaves": st["nleaf"], "depth": st["depth"],
                    "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"],

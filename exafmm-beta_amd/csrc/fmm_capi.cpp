@@ -116,7 +116,7 @@ struct fmm_plan {
   double cycle[3] = {0, 0, 0};
   fmm::M2LTiles tiles;    // prebuilt target tiles of those groups (TT > 0)
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
-  double t_bounds = 0, t_tree = 0, t_lists = 0;
+  double t_bounds = 0, t_tree = 0, t_lists = 0, t_groups = 0;
   int pair_targets = -1;  // fmm_plan_pair: bodies [0, pair_targets) are the targets
 };
 
@@ -339,6 +339,8 @@ void plan_stats(const fmm_plan *pl, fmm_stats &s) {
   s.host_bounds_ms = pl->t_bounds;
   s.host_tree_ms = pl->t_tree;
   s.host_lists_ms = pl->t_lists;
+  s.plan_groups_ms = pl->t_groups;
+  s.m2l_groups = pl->groups_kind >= 0 ? pl->groups.ngroup : 0;  // groups built with the plan
 }
 
 int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q);
@@ -411,9 +413,27 @@ int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, in
     }
     pl->lists = std::move(L);
     double t3 = now_ms();
+    // the M2L groups and tiles of the context's precision and order, when its
+    // M2L takes the tile path (else, or if they do not fit, built at upload)
+    const bool mixed = c->p.precision != FMM_FP64;
+    const bool tiles = c->p.m2l_mode == FMM_M2L_TILES ||
+                       (c->p.m2l_mode == FMM_M2L_AUTO && fmmdev::m2l_tiles_preferred(c->p.P, mixed));
+    if (tiles && (mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P))) {
+      bool done = false;
+      HIPCHK(fmmdev::device_groups_tiles(pl->tree, pl->lists, mixed, fmmdev::m2l_tile_targets(c->p.P), 16,
+                                         fmmdev::kM2LItemSlices, pl->groups, pl->tiles, done, c->st));
+      if (done) {
+        pl->groups_kind = mixed ? FMM_MIXED : FMM_FP64;
+      } else {
+        pl->groups = fmm::M2LGroups();
+        pl->tiles = fmm::M2LTiles();
+      }
+    }
+    double t4 = now_ms();
     pl->t_bounds = t1 - t0;
     pl->t_tree = t2 - t1;
     pl->t_lists = t3 - t2;
+    pl->t_groups = t4 - t3;
   } catch (const std::bad_alloc &) {
     FMM_FAIL(FMM_ERR_MEMORY, "host allocation failed while building the plan");
   }
@@ -786,7 +806,10 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   c->ntile = 0;
   fmm::M2LTiles tiles;
   if (c->use_mfma) {
-    if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64)) {
+    // groups built with the plan on the device carry no chunks (grp_ptr is
+    // {0}): the chunk path builds its own
+    if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64) &&
+        (c->use_tiles || pl.groups.grp_ptr.size() == (size_t)pl.groups.ngroup + 1)) {
       c->groups = std::move(pl.groups);
       if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) tiles = std::move(pl.tiles);
     } else {

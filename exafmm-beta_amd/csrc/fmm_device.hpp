@@ -14,6 +14,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "fmm_host.hpp"
+
 namespace fmmdev {
 
 struct DevBodies {
@@ -146,6 +148,14 @@ hipError_t launch_ewald(EwBodies B, int n, int ksize, int *kv, double2 *S, doubl
 constexpr int kMaxDevices = 64;
 constexpr int kEwaldScratchMaxK = 32;
 hipError_t init_scratch(int device);
+// The M2L groups and tiles of a regular free-space tree built on the device
+// (fmm_tiles.hip): the same partition into groups and classes as
+// fmm::build_m2l_groups (numbered in key order), the same tiles, slices and
+// work items as fmm::build_m2l_tiles; done = false (no error) when the plan
+// does not fit (the caller then builds them on the host).  Only the fields
+// the tile path reads are filled (no chunks).
+hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_level, int TT, int SW, int seg,
+                               fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s);
 // The octree of n bodies (host array x, AoS) built on the device: the same
 // cells, numbering and body order as fmm::build_tree for the root box (X0,
 // R0) (fmm_traverse.hip); outputs are host arrays.

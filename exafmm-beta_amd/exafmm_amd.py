@@ -91,7 +91,8 @@ class Stats(ctypes.Structure):
                 ("dev_m2l_ms", ctypes.c_double), ("dev_downward_ms", ctypes.c_double),
                 ("dev_total_ms", ctypes.c_double), ("dev_m2l_kernel_ms", ctypes.c_double),
                 ("dev_p2p_kernel_ms", ctypes.c_double), ("m2l_groups", ctypes.c_int64),
-                ("m2l_ungrouped", ctypes.c_int64), ("m2l_path", ctypes.c_int)]
+                ("m2l_ungrouped", ctypes.c_int64), ("m2l_path", ctypes.c_int),
+                ("plan_groups_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {}

@@ -91,6 +91,9 @@ typedef struct {
    * taken by the last upload (fmm_m2l_mode value actually used) */
   int64_t m2l_groups, m2l_ungrouped;
   int m2l_path;
+  /* M2L groups and tiles built with the plan on the device (fmm_plan_build_device;
+   * otherwise 0 and their host build is part of host_upload_ms) */
+  double plan_groups_ms;
 } fmm_stats;
 
 /* Fill p with the reference defaults (include/args.h:188-205). */

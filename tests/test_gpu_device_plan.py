@@ -101,3 +101,42 @@ def test_device_plan_full_size(dist, n):
           % (dist, n, s["m2l_pairs"], s["p2p_pairs"], hp.stats()["host_tree_ms"], dp.stats()["host_tree_ms"],
              hp.stats()["host_lists_ms"], dp.stats()["host_lists_ms"]))
     f.close()
+
+
+@pytest.mark.parametrize("P,prec,dist", [(10, "fp64", "plummer1"), (12, "fp64", "cube1"), (8, "mixed", "plummer1"),
+                                         (12, "mixed", "cube1"), (12, "mixed16", "plummer1"), (4, "fp64", "cube1")])
+def test_device_groups_and_tiles_solve_like_host(P, prec, dist):
+    """M2L groups and tiles built with the plan on the device: the same
+    groups (count) and the same solve as the host-built ones, to the rounding
+    of the order in which a tile adds its slices (1e-12)."""
+    F = _F()
+    x, q = F.make_bodies(40000, dist, 4)
+    pr = {"fp64": F.FMM_FP64, "mixed": F.FMM_MIXED, "mixed16": F.FMM_MIXED16}[prec]
+    f = F.FMM(P=P, precision=pr)
+    hp = F.Plan(x)
+    dp = F.Plan(x, lists_on=f)
+    sd = dp.stats()
+    assert sd["m2l_groups"] > 0  # built on the device
+    # the local expansions after the upward pass and M2L alone: the P2P of
+    # the mixed path sums each target's sources in FP32 runs, whose values
+    # follow the list order (source order on the device, visiting order on
+    # the host), so the whole solve is compared at FP64 only
+    out = {}
+    for name, plan in (("host", hp), ("device", dp)):
+        f.set_bodies_plan(plan, q)
+        f.upward()
+        f.m2l()
+        f.synchronize()
+        out[name] = (f.expansions()[1], f.stats())
+        if prec == "fp64":
+            f.evaluate()
+            out[name] += (f.targets(),)
+    f.close()
+    (Lh, sh), (Ld, st) = out["host"][:2], out["device"][:2]
+    assert st["m2l_path"] == F.M2L_TILES and sh["m2l_path"] == F.M2L_TILES
+    assert st["m2l_groups"] == sh["m2l_groups"] == sd["m2l_groups"] and st["m2l_ungrouped"] == 0
+    assert np.linalg.norm(Ld - Lh) / np.linalg.norm(Lh) < 1e-12
+    if prec == "fp64":
+        th, td = out["host"][2], out["device"][2]
+        rel = np.linalg.norm(td - th, axis=0) / np.linalg.norm(th, axis=0)
+        assert np.all(rel < 1e-12), rel
