@@ -118,6 +118,10 @@ struct fmm_plan {
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   double t_bounds = 0, t_tree = 0, t_lists = 0, t_groups = 0;
   int pair_targets = -1;  // fmm_plan_pair: bodies [0, pair_targets) are the targets
+  // fmm_plan_build_device: the lists' source columns and the tiles' pair
+  // arrays left on the device (copies of the plan share them); the host
+  // vectors of lists and tiles stay empty until a host-side use fetches them
+  std::shared_ptr<fmmdev::PlanArrays> dev;
 };
 
 #define FMM_FAIL(code, msg) \
@@ -157,6 +161,30 @@ int upload(fmm_ctx *c, DBuf &b, const T *h, size_t count) {
   if (rc) return rc;
   if (count) HIPCHK(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, c->st));
   return FMM_OK;
+}
+
+// a device-held plan array into a context buffer (same device)
+int dcopy(fmm_ctx *c, DBuf &b, const fmmdev::DevArray &a) {
+  int rc = ensure(c, b, a.bytes);
+  if (rc) return rc;
+  if (a.bytes) HIPCHK(hipMemcpyAsync(b.p, a.p, a.bytes, hipMemcpyDeviceToDevice, c->st));
+  return FMM_OK;
+}
+
+// the host copies of a device-built plan's lists and tiles, for host-side
+// uses (merging, host grouping, list export, another device); the plan's
+// contents do not change, only where they are held
+hipError_t plan_to_host(const fmm_plan *pl) {
+  if (!pl->dev || !(pl->dev->lists || pl->dev->tiles)) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != pl->dev->device && (e = hipSetDevice(pl->dev->device)) != hipSuccess) return e;
+  fmm_plan *m = const_cast<fmm_plan *>(pl);
+  e = fmmdev::fetch_lists(*pl->dev, m->lists, nullptr);
+  if (e == hipSuccess && m->tiles.TT > 0) e = fmmdev::fetch_tiles(*pl->dev, m->tiles, nullptr);
+  if (cur != pl->dev->device) (void)hipSetDevice(cur);
+  return e;
 }
 
 void release(DBuf &b) {
@@ -402,11 +430,13 @@ int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, in
     const fmm::Tree &t = pl->tree;
     fmm::Lists L = fmm::empty_lists(t.ncell);
     const double xper[3] = {0, 0, 0};
+    // the source columns stay on the device with the plan (pl->dev)
+    pl->dev = std::make_shared<fmmdev::PlanArrays>();
     HIPCHK(fmmdev::device_lists(t.ncell, t.X.data(), t.R.data(), t.nchild.data(), t.ichild.data(), t.nbody.data(),
                                 theta, nspawn, xper, 0, 0, L.m2l_ptr, L.m2l_src, L.p2p_ptr, L.p2p_src,
-                                L.p2p_body_pairs, c->st));
-    L.nm2l = (int64_t)L.m2l_src.size();
-    L.np2p = (int64_t)L.p2p_src.size();
+                                L.p2p_body_pairs, c->st, pl->dev.get()));
+    L.nm2l = t.ncell > 0 ? (int64_t)L.m2l_ptr[t.ncell] : 0;
+    L.np2p = t.ncell > 0 ? (int64_t)L.p2p_ptr[t.ncell] : 0;
     for (int i = 0; i < t.ncell; i++) {
       if (L.m2l_ptr[i + 1] > L.m2l_ptr[i]) L.m2l_targets.push_back(i);
       if (L.p2p_ptr[i + 1] > L.p2p_ptr[i]) L.p2p_targets.push_back(i);
@@ -421,7 +451,8 @@ int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, in
     if (tiles && (mixed ? fmmdev::m2l_mixed_supported(c->p.P) : fmmdev::m2l_mfma_supported(c->p.P))) {
       bool done = false;
       HIPCHK(fmmdev::device_groups_tiles(pl->tree, pl->lists, mixed, fmmdev::m2l_tile_targets(c->p.P), 16,
-                                         fmmdev::kM2LItemSlices, pl->groups, pl->tiles, done, c->st));
+                                         fmmdev::kM2LItemSlices, pl->groups, pl->tiles, done, c->st,
+                                         pl->dev.get()));
       if (done) {
         pl->groups_kind = mixed ? FMM_MIXED : FMM_FP64;
       } else {
@@ -512,6 +543,7 @@ int fmm_plan_pair(fmm_plan **out, const fmm_plan *targets, const fmm_plan *sourc
 int fmm_plan_build_groups(fmm_plan *pl, int precision) {
   if (!pl || (precision != FMM_FP64 && precision != FMM_MIXED && precision != FMM_MIXED16)) return FMM_ERR_ARG;
   if (precision == FMM_MIXED16) precision = FMM_MIXED;  // the same groups (by level)
+  if (plan_to_host(pl) != hipSuccess) return FMM_ERR_DEVICE;
   try {
     pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, precision == FMM_MIXED,
                                        pl->images > 0 ? pl->cycle : nullptr);
@@ -532,6 +564,7 @@ int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p) {
   if (!(mixed ? fmmdev::m2l_mixed_supported(p->P) : fmmdev::m2l_mfma_supported(p->P))) return FMM_OK;
   try {
     if (pl->groups_kind != kind) {
+      if (plan_to_host(pl) != hipSuccess) return FMM_ERR_DEVICE;
       pl->groups = fmm::build_m2l_groups(pl->tree, pl->lists, fmmdev::kM2LChunkPairs, mixed,
                                          pl->images > 0 ? pl->cycle : nullptr);
       pl->groups_kind = kind;
@@ -556,12 +589,17 @@ int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans) {
   int64_t n = 0;
   for (int i = 0; i < k; i++) {
     if (!plans[i] || plans[i]->images != 0) return FMM_ERR_ARG;  // forests of free-space systems only
+    if (plan_to_host(plans[i]) != hipSuccess) return FMM_ERR_DEVICE;
     n += plans[i]->n;
     parts.push_back({&plans[i]->tree, &plans[i]->lists, &plans[i]->groups, &plans[i]->tiles});
   }
   if (n > INT32_MAX) return FMM_ERR_ARG;
+  // groups built with the plan on the device hold no pair arrays (grp_ptr is
+  // {0}): a merge with them takes no groups, the upload builds its own
   bool same_kind = true;
-  for (int i = 0; i < k; i++) same_kind = same_kind && plans[i]->groups_kind == kind;
+  for (int i = 0; i < k; i++)
+    same_kind = same_kind && plans[i]->groups_kind == kind &&
+                plans[i]->groups.grp_ptr.size() == (size_t)plans[i]->groups.ngroup + 1;
   fmm_plan *pl = new (std::nothrow) fmm_plan();
   if (!pl) return FMM_ERR_MEMORY;
   try {
@@ -644,6 +682,7 @@ int fmm_get_list_counts(fmm_ctx *c, uint64_t *icell, int *nump2p, int *numm2l) {
 
 int fmm_plan_lists(const fmm_plan *pl, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
   if (!pl) return FMM_ERR_ARG;
+  if ((m2l_src || p2p_src) && plan_to_host(pl) != hipSuccess) return FMM_ERR_DEVICE;
   copy_lists(pl->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);
   return FMM_OK;
 }
@@ -678,6 +717,12 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   graph_reset(c);
   c->have_bodies = false;
   const double t3 = now_ms();
+  // a device-built plan's arrays are copied device to device on their own
+  // device, else from host copies
+  const bool on_dev = pl.dev && (pl.dev->lists || pl.dev->tiles) && pl.dev->device == c->p.device;
+  if (pl.dev && !on_dev) HIPCHK(plan_to_host(&pl));
+  std::shared_ptr<fmmdev::PlanArrays> pdev = on_dev ? pl.dev : nullptr;
+  const bool dev_lists = pdev && pdev->lists, dev_tiles = pdev && pdev->tiles;
   const int n = pl.n;
   c->tree = std::move(pl.tree);
   c->lists = std::move(pl.lists);
@@ -722,19 +767,29 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     (void)hipStreamSynchronize(c->st); /* staging vectors in use */ \
     return rc;                                                      \
   }
+#define DC(buf, arr)                                                \
+  if ((rc = dcopy(c, c->buf, (arr)))) {                             \
+    (void)hipStreamSynchronize(c->st); /* staging vectors in use */ \
+    return rc;                                                      \
+  }
   UP(x, hx) UP(y, hy) UP(z, hz) UP(q, hq)
   UP(cx, hcx) UP(cy, hcy) UP(cz, hcz)
   UP(ibody, t.ibody) UP(nbody, t.nbody) UP(ichild, t.ichild) UP(nchild, t.nchild)
   UP(iparent, t.iparent)
   UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
-  UP(m2l_ptr, L.m2l_ptr) UP(m2l_src, L.m2l_src) UP(m2l_tgt, L.m2l_targets)
+  UP(m2l_ptr, L.m2l_ptr) UP(m2l_tgt, L.m2l_targets)
+  if (dev_lists) {
+    DC(m2l_src, pdev->m2l_src) DC(p2p_src, pdev->p2p_src)
+  } else {
+    UP(m2l_src, L.m2l_src) UP(p2p_src, L.p2p_src)
+  }
   // P2P targets, leaves of >= 16 bodies first (the kernel's targets per lane
   // follow the leaf size, so every leaf is summed the same way in any batch)
   std::vector<int> p2p_order(L.p2p_targets);
   c->p2p_nwide = (int)(std::stable_partition(p2p_order.begin(), p2p_order.end(),
                                              [&](int cell) { return t.nbody[cell] >= fmmdev::kP2PWideLeaf; }) -
                        p2p_order.begin());
-  UP(p2p_ptr, L.p2p_ptr) UP(p2p_src, L.p2p_src) UP(p2p_tgt, p2p_order)
+  UP(p2p_ptr, L.p2p_ptr) UP(p2p_tgt, p2p_order)
   // M2M / L2L operators: every level's radius uniform and every child at
   // X_parent + R_child (+-1, +-1, +-1) (up to rounding of the centres)
   c->xl = fmmdev::xlate_mfma_supported(c->p.P) && nc > 1;
@@ -813,20 +868,33 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
       c->groups = std::move(pl.groups);
       if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) tiles = std::move(pl.tiles);
     } else {
+      if (dev_lists) {  // host grouping reads them
+        const hipError_t e = fmmdev::fetch_lists(*pdev, c->lists, c->st);
+        if (e != hipSuccess) {
+          (void)hipStreamSynchronize(c->st);
+          HIPCHK(e);
+        }
+      }
       c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed,
                                         c->p.images > 0 ? c->p.cycle : nullptr);
     }
     const fmm::M2LGroups &G = c->groups;
     UP(g_dx, G.dx) UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
     if (c->use_tiles) {
+      const bool dt = tiles.TT > 0 && dev_tiles;  // the plan's tiles, their pair arrays on this device
       if (tiles.TT == 0) tiles = fmm::build_m2l_tiles(t, G, fmmdev::m2l_tile_targets(c->p.P), 16);
       c->ntile = (int)(tiles.item.size() / 4);  // work items
       // the tile kernel holds an item's slice descriptors one per lane
       for (size_t i = 0; i < tiles.item.size(); i += 4)
         if (tiles.item[i + 2] - tiles.item[i + 1] > fmmdev::kM2LItemSlices)
           FMM_FAIL(FMM_ERR_UNSUPPORTED, "M2L tiles: a work item holds more than 64 slices");
-      UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item) UP(t_slice, tiles.slice) UP(t_src, tiles.src)
-      UP(t_lt, tiles.lt) UP(c_dx, G.cdx)
+      UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item)
+      if (dt) {
+        DC(t_slice, pdev->t_slice) DC(t_src, pdev->t_src) DC(t_lt, pdev->t_lt)
+      } else {
+        UP(t_slice, tiles.slice) UP(t_src, tiles.src) UP(t_lt, tiles.lt)
+      }
+      UP(c_dx, G.cdx)
       if (c->mixed) UP(c_sc, G.cscale)
     } else {
       c->nchunk = (int)(G.chunk.size() / 4);
@@ -837,6 +905,7 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     }
   }
 #undef UP
+#undef DC
   HIPCHK(hipStreamSynchronize(c->st));  // host staging vectors die here
   if ((rc = ensure(c, c->pot, sizeof(double) * n))) return rc;
   if ((rc = ensure(c, c->ax, sizeof(double) * n))) return rc;
@@ -1447,7 +1516,15 @@ int fmm_get_tree(fmm_ctx *c, int *perm, int *iparent, int *ichild, int *nchild, 
 int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p_src) {
   if (!c) return FMM_ERR_ARG;
   if (!c->have_bodies) FMM_FAIL(FMM_ERR_STATE, "no bodies");
-  copy_lists(c->lists, m2l_ptr, m2l_src, p2p_ptr, p2p_src);
+  const fmm::Lists &L = c->lists;
+  copy_lists(L, m2l_ptr, (int64_t)L.m2l_src.size() == L.nm2l ? m2l_src : nullptr, p2p_ptr,
+             (int64_t)L.p2p_src.size() == L.np2p ? p2p_src : nullptr);
+  // lists uploaded from a device-built plan: the source columns from the context's device copies
+  HIPCHK(hipSetDevice(c->p.device));
+  if (m2l_src && (int64_t)L.m2l_src.size() != L.nm2l)
+    HIPCHK(hipMemcpy(m2l_src, c->m2l_src.p, sizeof(int) * (size_t)L.nm2l, hipMemcpyDeviceToHost));
+  if (p2p_src && (int64_t)L.p2p_src.size() != L.np2p)
+    HIPCHK(hipMemcpy(p2p_src, c->p2p_src.p, sizeof(int) * (size_t)L.np2p, hipMemcpyDeviceToHost));
   return FMM_OK;
 }
 

@@ -148,14 +148,48 @@ hipError_t launch_ewald(EwBodies B, int n, int ksize, int *kv, double2 *S, doubl
 constexpr int kMaxDevices = 64;
 constexpr int kEwaldScratchMaxK = 32;
 hipError_t init_scratch(int device);
+// One device allocation (hipMalloc on `device`), freed on that device when
+// its owner goes; move-only.
+struct DevArray {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int device = -1;
+  DevArray() = default;
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  ~DevArray() { reset(); }
+  hipError_t alloc(size_t b, int dev);  // the device must be current
+  void reset();
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+// The large arrays of a plan built on the device, left there for the upload
+// into a context on the same device: the lists' source columns (the host
+// Lists keep the row pointers and counts, their m2l_src and p2p_src stay
+// empty) and the M2L tiles' slices, pair sources and pair targets (the host
+// M2LTiles keep the tiles, items and target cells).  Written on the
+// builder's stream, complete when the build returns.
+struct PlanArrays {
+  int device = -1;
+  bool lists = false, tiles = false;
+  DevArray m2l_src, p2p_src;    // int per pair
+  DevArray t_slice, t_src, t_lt;  // 4 ints per slice, int and uint16 per pair
+  int64_t nslice = 0;
+};
+// Copy the device-held parts into the host structures (synchronous on s).
+hipError_t fetch_lists(const PlanArrays &d, fmm::Lists &L, hipStream_t s);
+hipError_t fetch_tiles(const PlanArrays &d, fmm::M2LTiles &T, hipStream_t s);
 // The M2L groups and tiles of a regular free-space tree built on the device
 // (fmm_tiles.hip): the same partition into groups and classes as
 // fmm::build_m2l_groups (numbered in key order), the same tiles, slices and
 // work items as fmm::build_m2l_tiles; done = false (no error) when the plan
 // does not fit (the caller then builds them on the host).  Only the fields
-// the tile path reads are filled (no chunks).
+// the tile path reads are filled (no chunks).  With keep, the pair sources
+// are read from keep->m2l_src when keep->lists (else from L.m2l_src), and
+// the tiles' slices, pair sources and pair targets are left in keep instead
+// of T (keep->tiles).
 hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_level, int TT, int SW, int seg,
-                               fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s);
+                               fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s,
+                               PlanArrays *keep = nullptr);
 // The octree of n bodies (host array x, AoS) built on the device: the same
 // cells, numbering and body order as fmm::build_tree for the root box (X0,
 // R0) (fmm_traverse.hip); outputs are host arrays.
@@ -165,11 +199,14 @@ hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], do
                        std::vector<double> &X, std::vector<double> &R, hipStream_t s);
 // The interaction lists of a free-space tree built on the device (the dual
 // traversal breadth first, fmm_traverse.hip): CSR lists sorted by (target,
-// source), the P2P body-pair count; the cell arrays are host arrays.
+// source), the P2P body-pair count; the cell arrays are host arrays.  With
+// keep, the source columns stay on the device in keep->m2l_src and
+// keep->p2p_src (keep->lists) and m2l_src, p2p_src are left empty.
 hipError_t device_lists(int ncell, const double *X, const double *R, const int *nchild, const int *ichild,
                         const int *nbody, double theta, int nspawn, const double xper[3], int root_i, int root_j,
                         std::vector<int> &m2l_ptr, std::vector<int> &m2l_src, std::vector<int> &p2p_ptr,
-                        std::vector<int> &p2p_src, int64_t &body_pairs, hipStream_t s);
+                        std::vector<int> &p2p_src, int64_t &body_pairs, hipStream_t s,
+                        PlanArrays *keep = nullptr);
 hipError_t launch_ewald_scratch(int device, EwBodies B, int n, int ksize, double3 cyc, double alpha, double sigma,
                                 double cutoff, hipStream_t s);
 hipError_t launch_direct(DevBodies B, int n, const double *tx, const double *ty, const double *tz, int nt,

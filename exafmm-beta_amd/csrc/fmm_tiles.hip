@@ -248,8 +248,16 @@ __global__ void slice_desc_kernel(int nseg, const unsigned long long *__restrict
 }  // namespace
 
 hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_level, int TT, int SW, int seg,
-                               fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s) {
+                               fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s,
+                               PlanArrays *keep) {
   done = false;
+  if (keep) {
+    keep->tiles = false;
+    keep->t_slice.reset();
+    keep->t_src.reset();
+    keep->t_lt.reset();
+    keep->nslice = 0;
+  }
   const int nc = t.ncell;
   const int64_t np = L.nm2l;
   if (nc == 0 || np == 0 || np > 0x7fffffff || TT > 255) return hipSuccess;
@@ -261,13 +269,16 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   TCHK(dlev.need(4 * (size_t)nc));
   TCHK(dpar.need(4 * (size_t)nc));
   TCHK(dptr.need(4 * ((size_t)nc + 1)));
-  TCHK(dsrc.need(4 * (size_t)np));
+  // the pair sources: the lists' device column when the plan keeps one
+  const bool dev_src = keep && keep->lists && keep->m2l_src.bytes == 4 * (size_t)np;
+  if (!dev_src) TCHK(dsrc.need(4 * (size_t)np));
+  const int *psrc = dev_src ? keep->m2l_src.as<int>() : dsrc.as<int>();
   TCHK(dtgt.need(4 * (size_t)np));
   TCHK(hipMemcpyAsync(dX.p, t.X.data(), sizeof(double) * 3 * (size_t)nc, hipMemcpyHostToDevice, s));
   TCHK(hipMemcpyAsync(dlev.p, t.level.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, s));
   TCHK(hipMemcpyAsync(dpar.p, t.iparent.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, s));
   TCHK(hipMemcpyAsync(dptr.p, L.m2l_ptr.data(), 4 * ((size_t)nc + 1), hipMemcpyHostToDevice, s));
-  TCHK(hipMemcpyAsync(dsrc.p, L.m2l_src.data(), 4 * (size_t)np, hipMemcpyHostToDevice, s));
+  if (!dev_src) TCHK(hipMemcpyAsync(dsrc.p, L.m2l_src.data(), 4 * (size_t)np, hipMemcpyHostToDevice, s));
   const unsigned nbc = (unsigned)((nc + 255) / 256), nbp = (unsigned)((np + 255) / 256);
   pair_target_kernel<<<nbc, 256, 0, s>>>(nc, dptr.as<int>(), dtgt.as<int>());
   TCHK(hipGetLastError());
@@ -278,7 +289,7 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   TCHK(dbad.need(4));
   TCHK(hipMemsetAsync(dtab.p, 0xff, 8 * tsize, s));
   TCHK(hipMemsetAsync(dbad.p, 0, 4, s));
-  pair_key_kernel<<<nbp, 256, 0, s>>>(np, dtgt.as<int>(), dsrc.as<int>(), dX.as<double>(), dlev.as<int>(), R0,
+  pair_key_kernel<<<nbp, 256, 0, s>>>(np, dtgt.as<int>(), psrc, dX.as<double>(), dlev.as<int>(), R0,
                                        by_level ? 1 : 0, dtab.as<unsigned long long>(), dslot.as<int>(),
                                        dbad.as<int>());
   TCHK(hipGetLastError());
@@ -437,28 +448,57 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   int nslice = 0;
   TCHK(hipMemcpyAsync(&nslice, dsb.as<int>() + nseg, 4, hipMemcpyDeviceToHost, s));
   TCHK(hipStreamSynchronize(s));
+  // the tiles' pair sources, pair targets and slices: on the device in keep,
+  // else staged here and copied to T
   DBufT dts(s), dtl(s), dsd(s), dtp(s);
-  TCHK(dts.need(4 * (size_t)np));
-  TCHK(dtl.need(2 * (size_t)np));
-  TCHK(dsd.need(16 * (size_t)nslice));
+  int *osrc, *osd;
+  unsigned short *olt;
+  if (keep) {
+    int dev = 0;
+    TCHK(hipGetDevice(&dev));
+    TCHK(keep->t_src.alloc(4 * (size_t)np, dev));
+    TCHK(keep->t_lt.alloc(2 * (size_t)np, dev));
+    TCHK(keep->t_slice.alloc(16 * (size_t)nslice, dev));
+    osrc = keep->t_src.as<int>();
+    olt = keep->t_lt.as<unsigned short>();
+    osd = keep->t_slice.as<int>();
+  } else {
+    TCHK(dts.need(4 * (size_t)np));
+    TCHK(dtl.need(2 * (size_t)np));
+    TCHK(dsd.need(16 * (size_t)nslice));
+    osrc = dts.as<int>();
+    olt = dtl.as<unsigned short>();
+    osd = dsd.as<int>();
+  }
   TCHK(dtp.need(4 * ((size_t)T.ntile + 1)));
   deal_kernel<<<nbp, 256, 0, s>>>(np, dpk2.as<unsigned long long>(), dpv2.as<int>(), dsegid.as<int>(), dss.as<int>(),
-                                  dsl.as<int>(), dns.as<int>(), dsrc.as<int>(), dslot.as<int>(), ds2g.as<int>(),
-                                  dgr.as<unsigned char>(), dts.as<int>(), dtl.as<unsigned short>());
+                                  dsl.as<int>(), dns.as<int>(), psrc, dslot.as<int>(), ds2g.as<int>(),
+                                  dgr.as<unsigned char>(), osrc, olt);
   TCHK(hipGetLastError());
   slice_desc_kernel<<<nbs, 256, 0, s>>>(nseg, dpk2.as<unsigned long long>(), dss.as<int>(), dsl.as<int>(),
-                                        dns.as<int>(), dsb.as<int>(), dsd.as<int>(), dtp.as<int>());
+                                        dns.as<int>(), dsb.as<int>(), osd, dtp.as<int>());
   TCHK(hipGetLastError());
   T.ptr.resize((size_t)T.ntile + 1);
-  T.slice.resize(4 * (size_t)nslice);
-  T.src.resize(np);
-  T.lt.resize(np);
   TCHK(hipMemcpyAsync(T.ptr.data(), dtp.p, 4 * (size_t)T.ntile, hipMemcpyDeviceToHost, s));
-  TCHK(hipMemcpyAsync(T.slice.data(), dsd.p, 16 * (size_t)nslice, hipMemcpyDeviceToHost, s));
-  TCHK(hipMemcpyAsync(T.src.data(), dts.p, 4 * (size_t)np, hipMemcpyDeviceToHost, s));
-  TCHK(hipMemcpyAsync(T.lt.data(), dtl.p, 2 * (size_t)np, hipMemcpyDeviceToHost, s));
+  if (keep) {
+    T.slice.clear();
+    T.src.clear();
+    T.lt.clear();
+  } else {
+    T.slice.resize(4 * (size_t)nslice);
+    T.src.resize(np);
+    T.lt.resize(np);
+    TCHK(hipMemcpyAsync(T.slice.data(), dsd.p, 16 * (size_t)nslice, hipMemcpyDeviceToHost, s));
+    TCHK(hipMemcpyAsync(T.src.data(), dts.p, 4 * (size_t)np, hipMemcpyDeviceToHost, s));
+    TCHK(hipMemcpyAsync(T.lt.data(), dtl.p, 2 * (size_t)np, hipMemcpyDeviceToHost, s));
+  }
   TCHK(hipStreamSynchronize(s));
   T.ptr[T.ntile] = nslice;
+  if (keep) {
+    TCHK(hipGetDevice(&keep->device));
+    keep->nslice = nslice;
+    keep->tiles = true;
+  }
   fmm::split_m2l_tiles(T, seg);
   done = true;
   return hipSuccess;

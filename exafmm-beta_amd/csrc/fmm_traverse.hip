@@ -191,11 +191,12 @@ struct Buf {
   } while (0)
 
 // sorted CSR of a pair list on the device: ptr (ncell + 1) and the sources,
-// copied to the host
+// copied to the host, or, with keep, left on the device in keep (src empty)
 hipError_t pairs_to_csr(const int2 *pr, size_t n, int ncell, std::vector<int> &ptr, std::vector<int> &src,
-                        hipStream_t s) {
+                        hipStream_t s, DevArray *keep) {
   ptr.assign((size_t)ncell + 1, 0);
-  src.resize(n);
+  src.clear();
+  if (!keep) src.resize(n);
   Buf keys(s), keys2(s), cnt(s), tmp(s), out(s);
   TCHK(cnt.need(sizeof(int) * ((size_t)ncell + 1)));
   TCHK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)ncell + 1), s));
@@ -215,10 +216,21 @@ hipError_t pairs_to_csr(const int2 *pr, size_t n, int ncell, std::vector<int> &p
     TCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, db, (int)n, 0, hi_bit, s));
     TCHK(tmp.need(tb));
     TCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, db, (int)n, 0, hi_bit, s));
-    TCHK(out.need(sizeof(int) * n));
-    key_src_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(db.Current(), n, out.as<int>());
+    int *dst;
+    if (keep) {
+      int dev = 0;
+      TCHK(hipGetDevice(&dev));
+      TCHK(keep->alloc(sizeof(int) * n, dev));
+      dst = keep->as<int>();
+    } else {
+      TCHK(out.need(sizeof(int) * n));
+      dst = out.as<int>();
+    }
+    key_src_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(db.Current(), n, dst);
     TCHK(hipGetLastError());
-    TCHK(hipMemcpyAsync(src.data(), out.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    if (!keep) TCHK(hipMemcpyAsync(src.data(), out.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  } else if (keep) {
+    keep->reset();
   }
   std::vector<int> c((size_t)ncell + 1);
   TCHK(hipMemcpyAsync(c.data(), cnt.p, sizeof(int) * ((size_t)ncell + 1), hipMemcpyDeviceToHost, s));
@@ -437,7 +449,13 @@ __global__ void tree_leafkey_kernel(int n, const int *__restrict__ idx, const in
 hipError_t device_lists(int ncell, const double *X, const double *R, const int *nchild, const int *ichild,
                         const int *nbody, double theta, int nspawn, const double xper[3], int root_i, int root_j,
                         std::vector<int> &m2l_ptr, std::vector<int> &m2l_src, std::vector<int> &p2p_ptr,
-                        std::vector<int> &p2p_src, int64_t &body_pairs, hipStream_t s) {
+                        std::vector<int> &p2p_src, int64_t &body_pairs, hipStream_t s, PlanArrays *keep) {
+  if (keep) {
+    keep->lists = false;
+    keep->m2l_src.reset();
+    keep->p2p_src.reset();
+    TCHK(hipGetDevice(&keep->device));
+  }
   m2l_ptr.assign((size_t)ncell + 1, 0);
   p2p_ptr.assign((size_t)ncell + 1, 0);
   m2l_src.clear();
@@ -510,9 +528,65 @@ hipError_t device_lists(int ncell, const double *X, const double *R, const int *
     TCHK(hipStreamSynchronize(s));
     body_pairs = (int64_t)v;
   }
-  TCHK(pairs_to_csr(M.as<int2>(), nm, ncell, m2l_ptr, m2l_src, s));
-  TCHK(pairs_to_csr(Pp.as<int2>(), np, ncell, p2p_ptr, p2p_src, s));
+  TCHK(pairs_to_csr(M.as<int2>(), nm, ncell, m2l_ptr, m2l_src, s, keep ? &keep->m2l_src : nullptr));
+  TCHK(pairs_to_csr(Pp.as<int2>(), np, ncell, p2p_ptr, p2p_src, s, keep ? &keep->p2p_src : nullptr));
+  if (keep) keep->lists = true;
   return hipSuccess;
+}
+
+hipError_t DevArray::alloc(size_t b, int dev) {
+  reset();
+  if (b == 0) return hipSuccess;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = b;
+  device = dev;
+  return hipSuccess;
+}
+
+void DevArray::reset() {
+  if (p) {
+    int cur = -1;
+    const bool same = hipGetDevice(&cur) == hipSuccess && cur == device;
+    if (!same) (void)hipSetDevice(device);
+    (void)hipFree(p);
+    if (!same && cur >= 0) (void)hipSetDevice(cur);
+  }
+  p = nullptr;
+  bytes = 0;
+  device = -1;
+}
+
+hipError_t fetch_lists(const PlanArrays &d, fmm::Lists &L, hipStream_t s) {
+  if (!d.lists) return hipSuccess;
+  if ((int64_t)L.m2l_src.size() != L.nm2l) {
+    L.m2l_src.resize(L.nm2l);
+    if (L.nm2l) TCHK(hipMemcpyAsync(L.m2l_src.data(), d.m2l_src.p, 4 * (size_t)L.nm2l, hipMemcpyDeviceToHost, s));
+  }
+  if ((int64_t)L.p2p_src.size() != L.np2p) {
+    L.p2p_src.resize(L.np2p);
+    if (L.np2p) TCHK(hipMemcpyAsync(L.p2p_src.data(), d.p2p_src.p, 4 * (size_t)L.np2p, hipMemcpyDeviceToHost, s));
+  }
+  return hipStreamSynchronize(s);
+}
+
+hipError_t fetch_tiles(const PlanArrays &d, fmm::M2LTiles &T, hipStream_t s) {
+  if (!d.tiles) return hipSuccess;
+  const size_t np = d.t_src.bytes / 4;
+  if (T.src.size() != np) {
+    T.slice.resize(4 * (size_t)d.nslice);
+    T.src.resize(np);
+    T.lt.resize(np);
+    if (d.nslice) TCHK(hipMemcpyAsync(T.slice.data(), d.t_slice.p, 16 * (size_t)d.nslice, hipMemcpyDeviceToHost, s));
+    if (np) {
+      TCHK(hipMemcpyAsync(T.src.data(), d.t_src.p, 4 * np, hipMemcpyDeviceToHost, s));
+      TCHK(hipMemcpyAsync(T.lt.data(), d.t_lt.p, 2 * np, hipMemcpyDeviceToHost, s));
+    }
+  }
+  return hipStreamSynchronize(s);
 }
 
 hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], double R0, std::vector<int> &perm,

@@ -586,8 +586,10 @@ class Plan:
         p2p_ptr = np.zeros(nc + 1, np.int32)
         m2l_src = np.zeros(max(s["m2l_pairs"], 1), np.int32)
         p2p_src = np.zeros(max(s["p2p_pairs"], 1), np.int32)
-        load().fmm_plan_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip), _ptr(p2p_ptr, _ip),
-                              _ptr(p2p_src, _ip))
+        rc = load().fmm_plan_lists(self._h, _ptr(m2l_ptr, _ip), _ptr(m2l_src, _ip), _ptr(p2p_ptr, _ip),
+                                   _ptr(p2p_src, _ip))
+        if rc != 0:
+            raise FMMError("fmm_plan_lists failed (rc=%d)" % rc)
         return m2l_ptr, m2l_src[:s["m2l_pairs"]], p2p_ptr, p2p_src[:s["p2p_pairs"]]
 
     def list_counts(self):

@@ -121,8 +121,15 @@ int fmm_plan_build(fmm_plan **out, int n, const double *x, int ncrit, double the
  * body order as fmm_plan_build.  The lists: the dual traversal breadth first,
  * each round testing every frontier pair with the same acceptance test and
  * split rules -- the same pairs as fmm_plan_build, each cell's lists ordered
- * by source cell (the host lists keep the walk's visiting order).  The
- * bounds are computed on the host (their sequential 1e-5 drift). */
+ * by source cell (the host lists keep the walk's visiting order).  When c's
+ * M2L takes target tiles, the M2L groups and tiles are built there too (the
+ * same as fmm_plan_prepare would build for c).  The lists' source columns and
+ * the tiles' pair arrays stay in c's device memory with the plan (freed by
+ * fmm_plan_destroy): fmm_set_bodies_plan into a context on that device
+ * copies them device to device, and the host functions that read them
+ * (fmm_plan_lists, fmm_plan_merge, fmm_plan_prepare, a context on another
+ * device) copy them to the host first.  The bounds are computed on the host
+ * (their sequential 1e-5 drift). */
 int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, int ncrit, double theta,
                           int nspawn);
 void fmm_plan_destroy(fmm_plan *pl);
@@ -208,7 +215,8 @@ int fmm_plan_prepare(fmm_plan *pl, const fmm_params *p);
  * k plans (cells, bodies and pairs of plan i follow those of plans 0..i-1; no
  * interaction crosses systems).  Bodies of the merged plan are the k input
  * arrays concatenated in order.  Prebuilt groups are merged when every plan
- * has the same kind.  No counterpart in the reference (one system per solve
+ * has the same kind and was grouped on the host (groups built on the device
+ * hold no pair arrays; the merged plan is then grouped at upload).  No counterpart in the reference (one system per solve
  * there); host only. */
 int fmm_plan_merge(fmm_plan **out, int k, const fmm_plan *const *plans);
 

@@ -140,3 +140,65 @@ def test_device_groups_and_tiles_solve_like_host(P, prec, dist):
         th, td = out["host"][2], out["device"][2]
         rel = np.linalg.norm(td - th, axis=0) / np.linalg.norm(th, axis=0)
         assert np.all(rel < 1e-12), rel
+
+
+def _same_pairs(h, d):
+    """Lists (ptr, src, ptr, src) holding the same pairs, the device ones
+    sorted by source within each target."""
+    for hptr, hsrc, dptr, dsrc in ((h[0], h[1], d[0], d[1]), (h[2], h[3], d[2], d[3])):
+        assert np.array_equal(hptr, dptr)
+        tgt = np.repeat(np.arange(len(hptr) - 1), np.diff(hptr))
+        assert np.array_equal(hsrc[np.lexsort((hsrc, tgt))], dsrc)
+
+
+def test_device_plan_arrays_held_on_the_device():
+    """A device-built plan keeps its lists' source columns and its tiles'
+    pair arrays on the device, and a context on that device copies them
+    device to device.  Checked: the context's lists (read back from its
+    device copies) hold the host plan's pairs; the plan can go right after
+    the upload; a context of another kind (mixed precision, P = 8, from a
+    plan grouped for FP64 P = 10) groups from the lists fetched to the host
+    and matches the host plan after M2L (1e-12); a merge of device plans
+    (host copies fetched) solves like the merge of the host plans (1e-12)."""
+    F = _F()
+    x, q = F.make_bodies(30000, "plummer1", 5)
+    x2, q2 = F.make_bodies(12000, "cube1", 6)
+    f = F.FMM(P=10)
+    hp = F.Plan(x)
+    dp = F.Plan(x, lists_on=f)
+    assert dp.stats()["m2l_groups"] > 0
+    f.set_bodies_plan(dp, q)
+    del dp
+    _same_pairs(hp.lists(), f.lists())
+    f.evaluate()
+    td = f.targets()
+    f.set_bodies_plan(hp, q)
+    f.evaluate()
+    th = f.targets()
+    rel = np.linalg.norm(td - th, axis=0) / np.linalg.norm(th, axis=0)
+    assert np.all(rel < 1e-12), rel
+    # another kind of context: host grouping from the fetched lists
+    g = F.FMM(P=8, precision=F.FMM_MIXED)
+    dp = F.Plan(x, lists_on=f)
+    L = {}
+    for name, plan in (("host", hp), ("device", dp)):
+        g.set_bodies_plan(plan, q)
+        g.upward()
+        g.m2l()
+        g.synchronize()
+        L[name] = g.expansions()[1]
+    assert np.linalg.norm(L["device"] - L["host"]) / np.linalg.norm(L["host"]) < 1e-12
+    g.close()
+    # merged device plans against merged host plans
+    dp2, hp2 = F.Plan(x2, lists_on=f), F.Plan(x2)
+    md, mh = F.Plan.merge([dp, dp2]), F.Plan.merge([hp, hp2])
+    qq = np.concatenate([q, q2])
+    out = []
+    for m in (mh, md):
+        f.set_bodies_plan(m, qq)
+        f.evaluate()
+        out.append(f.targets())
+    f.close()
+    rel = np.linalg.norm(out[1] - out[0], axis=0) / np.linalg.norm(out[0], axis=0)
+    assert np.all(rel < 1e-12), rel
+    _same_pairs(mh.lists(), md.lists())
