@@ -78,6 +78,7 @@ struct fmm_ctx {
   std::vector<int> level_ptr;     // per level offsets into level_cells
   // device buffers
   DBuf x, y, z, q, pot, ax, ay, az;
+  DBuf xin, qin, perm;  // upload staging: bodies as given, the tree's permutation
   DBuf cx, cy, cz, ibody, nbody, ichild, nchild, iparent;
   DBuf M, L;
   DBuf leaves, internal, level_cells, body_leaf;
@@ -730,15 +731,6 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   const fmm::Lists &L = c->lists;
   c->n = n;
   const int nc = t.ncell;
-  // bodies in tree order (SoA)
-  std::vector<double> hx(n), hy(n), hz(n), hq(n);
-  for (int i = 0; i < n; i++) {
-    const int b = t.perm[i];
-    hx[i] = x[3 * (size_t)b];
-    hy[i] = x[3 * (size_t)b + 1];
-    hz[i] = x[3 * (size_t)b + 2];
-    hq[i] = q[b];
-  }
   std::vector<double> hcx(nc), hcy(nc), hcz(nc);
   for (int k = 0; k < nc; k++) {
     hcx[k] = t.X[3 * (size_t)k];
@@ -757,9 +749,6 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     }
   }
   c->internal_ptr[t.depth + 1] = (int)internal.size();
-  std::vector<int> body_leaf(n);
-  for (int leaf : t.leaves)
-    for (int i = t.ibody[leaf]; i < t.ibody[leaf] + t.nbody[leaf]; i++) body_leaf[i] = leaf;
   const size_t NT = nterm(c);
   int rc = FMM_OK;
 #define UP(buf, vec)                                                \
@@ -772,11 +761,26 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
     (void)hipStreamSynchronize(c->st); /* staging vectors in use */ \
     return rc;                                                      \
   }
-  UP(x, hx) UP(y, hy) UP(z, hz) UP(q, hq)
+  // bodies as given and the tree's permutation; gathered into tree order
+  // (SoA) on the device
+  if ((rc = upload(c, c->xin, x, 3 * (size_t)n)) || (rc = upload(c, c->qin, q, (size_t)n)) ||
+      (rc = upload(c, c->perm, t.perm.data(), t.perm.size())) || (rc = ensure(c, c->x, sizeof(double) * n)) ||
+      (rc = ensure(c, c->y, sizeof(double) * n)) || (rc = ensure(c, c->z, sizeof(double) * n)) ||
+      (rc = ensure(c, c->q, sizeof(double) * n)))
+    return rc;
+  HIPCHK(fmmdev::launch_gather_bodies(n, c->perm.as<int>(), c->xin.as<double>(), c->qin.as<double>(),
+                                      c->x.as<double>(), c->y.as<double>(), c->z.as<double>(), c->q.as<double>(),
+                                      c->st));
   UP(cx, hcx) UP(cy, hcy) UP(cz, hcz)
   UP(ibody, t.ibody) UP(nbody, t.nbody) UP(ichild, t.ichild) UP(nchild, t.nchild)
   UP(iparent, t.iparent)
-  UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells) UP(body_leaf, body_leaf)
+  UP(leaves, t.leaves) UP(internal, internal) UP(level_cells, t.level_cells)
+  if ((rc = ensure(c, c->body_leaf, sizeof(int) * n))) {
+    (void)hipStreamSynchronize(c->st);
+    return rc;
+  }
+  HIPCHK(fmmdev::launch_body_leaf((int)t.leaves.size(), c->leaves.as<int>(), c->ibody.as<int>(), c->nbody.as<int>(),
+                                  c->body_leaf.as<int>(), c->st));
   UP(m2l_ptr, L.m2l_ptr) UP(m2l_tgt, L.m2l_targets)
   if (dev_lists) {
     DC(m2l_src, pdev->m2l_src) DC(p2p_src, pdev->p2p_src)

@@ -32,6 +32,13 @@ constexpr int kMaxP = 20;
 
 // p[0, n) = 0 with a kernel (no memset node when the stream is being captured)
 hipError_t launch_zero(double *p, size_t n, hipStream_t s);
+// bodies into tree order: x/y/z/q[i] from input body perm[i] of xin (AoS)
+// and qin
+hipError_t launch_gather_bodies(int n, const int *perm, const double *xin, const double *qin, double *x, double *y,
+                                double *z, double *q, hipStream_t s);
+// body_leaf[i] = the leaf holding body i (tree order)
+hipError_t launch_body_leaf(int nleaf, const int *leaves, const int *ibody, const int *nbody, int *body_leaf,
+                            hipStream_t s);
 
 bool order_supported(int P);
 hipError_t init_constants();
