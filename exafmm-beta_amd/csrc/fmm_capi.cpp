@@ -326,7 +326,10 @@ void fmm_destroy(fmm_ctx *c) {
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->st2) (void)hipStreamDestroy(c->st2);
-  if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st) {
+    fmmdev::plan_scratch_release(c->st);  // plan builds on this stream
+    (void)hipStreamDestroy(c->st);
+  }
   delete c;  // the device buffers are freed here, with the device current
 }
 

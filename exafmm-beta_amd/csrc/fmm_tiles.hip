@@ -43,23 +43,21 @@ constexpr unsigned long long kEmptyKey = ~0ull;
 constexpr int kKBits = 14, kKOff = 1 << (kKBits - 1);  // offset components in [-2^13, 2^13)
 constexpr int kHashBits = 20;
 
-struct DBufT {  // device scratch from the stream-ordered pool
+struct DBufT {  // device scratch from the plan builders' per-stream cache
   hipStream_t s;
   void *p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // capacity
   explicit DBufT(hipStream_t st) : s(st) {}
   DBufT(const DBufT &) = delete;
   ~DBufT() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) plan_scratch_put(p, bytes, s);
   }
   hipError_t need(size_t b) {
     if (b <= bytes) return hipSuccess;
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) plan_scratch_put(p, bytes, s);
     p = nullptr;
     bytes = 0;
-    hipError_t e = hipMallocAsync(&p, b ? b : 1, s);
-    if (e == hipSuccess) bytes = b;
-    return e;
+    return plan_scratch_get(&p, &bytes, b, s);
   }
   template <class T> T *as() const { return static_cast<T *>(p); }
 };

@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -157,25 +159,22 @@ __global__ void body_pairs_kernel(const int2 *__restrict__ pr, size_t n, const i
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
 
-// device scratch from the stream-ordered pool: allocation and release are
-// queued on the stream, without the device-wide synchronisation of hipFree
+// device scratch from the plan builders' per-stream cache (plan_scratch_get)
 struct Buf {
   hipStream_t s;
   void *p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // capacity
   explicit Buf(hipStream_t st) : s(st) {}
   Buf(const Buf &) = delete;
   ~Buf() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) plan_scratch_put(p, bytes, s);
   }
   hipError_t need(size_t b) {
     if (b <= bytes) return hipSuccess;
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) plan_scratch_put(p, bytes, s);
     p = nullptr;
     bytes = 0;
-    hipError_t e = hipMallocAsync(&p, b, s);
-    if (e == hipSuccess) bytes = b;
-    return e;
+    return plan_scratch_get(&p, &bytes, b, s);
   }
   void swap(Buf &o) {
     std::swap(p, o.p);
@@ -532,6 +531,71 @@ hipError_t device_lists(int ncell, const double *X, const double *R, const int *
   TCHK(pairs_to_csr(Pp.as<int2>(), np, ncell, p2p_ptr, p2p_src, s, keep ? &keep->p2p_src : nullptr));
   if (keep) keep->lists = true;
   return hipSuccess;
+}
+
+namespace {
+struct ScratchCache {
+  std::multimap<size_t, void *> blocks;  // capacity -> block
+  size_t total = 0;
+};
+std::mutex g_scratch_mu;
+std::map<hipStream_t, ScratchCache> g_scratch;
+constexpr size_t kScratchKeep = (size_t)8 << 30;
+}  // namespace
+
+hipError_t plan_scratch_get(void **p, size_t *cap, size_t bytes, hipStream_t s) {
+  *p = nullptr;
+  *cap = 0;
+  if (bytes == 0) bytes = 1;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto f = g_scratch.find(s);
+    if (f != g_scratch.end()) {
+      auto it = f->second.blocks.lower_bound(bytes);
+      if (it != f->second.blocks.end() && it->first <= 2 * bytes + ((size_t)1 << 20)) {
+        *p = it->second;
+        *cap = it->first;
+        f->second.total -= it->first;
+        f->second.blocks.erase(it);
+        return hipSuccess;
+      }
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return e;
+  }
+  *cap = bytes;
+  return hipSuccess;
+}
+
+void plan_scratch_put(void *p, size_t cap, hipStream_t s) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchCache &c = g_scratch[s];
+    if (c.total + cap <= kScratchKeep) {
+      c.blocks.emplace(cap, p);
+      c.total += cap;
+      return;
+    }
+  }
+  (void)hipFree(p);  // waits for the device's work, the stream's included
+}
+
+void plan_scratch_release(hipStream_t s) {
+  ScratchCache c;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto f = g_scratch.find(s);
+    if (f == g_scratch.end()) return;
+    c = std::move(f->second);
+    g_scratch.erase(f);
+  }
+  if (c.blocks.empty()) return;
+  (void)hipStreamSynchronize(s);
+  for (auto &b : c.blocks) (void)hipFree(b.second);
 }
 
 hipError_t DevArray::alloc(size_t b, int dev) {
