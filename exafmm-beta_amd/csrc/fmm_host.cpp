@@ -18,17 +18,102 @@
 
 namespace fmm {
 
+namespace {
+
+// The scan step of getBounds (include/bound_box.h:16-23) for one coordinate:
+// m <- (v < m - 1e-5) ? v : m - 1e-5 (low side), m <- (v > m + 1e-5) ? v :
+// m + 1e-5 (high side).  Both are monotone in m and commute with min / max,
+// so a run of bodies acts on the incoming value as m -> min(c, s^k(m)), c
+// being the run's scan from +inf (-inf) and s^k the k-fold rounded shift: runs
+// are scanned in parallel and chained in order, with the same result, bit for
+// bit, as the one sequential scan.
+
+// s^k(a): k times a - 1e-5 (up: a + 1e-5), each rounded to double.  Inside one
+// binade the grid spacing u is fixed and a grid value moves by the same
+// rounded step d = -+round_u(1e-5) (1e-5 rounds without a tie on every grid
+// of spacing 2^-67 and above, i.e. every binade wide enough for a jump), so
+// j steps are the grid point a + j d, computed exactly by one fma.  Near a
+// binade edge, or where the steps are not yet inside one binade, steps are
+// taken one at a time.
+double drift(double a, int64_t k, bool up) {
+  const double c = 1e-5;
+  while (k > 0) {
+    const double b = up ? a + c : a - c;
+    --k;
+    if (k == 0 || !std::isfinite(b)) return b;
+    const double d = b - a;
+    if (d == 0) return b;  // a -+ 1e-5 rounds to a: fixed from here on
+    int ea = 0, eb = 0;
+    (void)std::frexp(a, &ea);
+    (void)std::frexp(b, &eb);
+    if (std::isnormal(a) && std::isnormal(b) && (a > 0) == (b > 0) && ea == eb) {
+      // |x| in [bot, top) on this binade; room for steps that keep both the
+      // point and its exact shifted value inside it (two steps to spare)
+      const double top = std::ldexp(1.0, eb), bot = std::ldexp(1.0, eb - 1);
+      const bool grow = (d > 0) == (b > 0);
+      const double ab = std::fabs(b), ad = std::fabs(d);
+      const double room = grow ? top - ab - c : ab - c - bot;
+      const double jj = std::floor(room / ad) - 2;
+      if (jj >= 1) {
+        const int64_t j = (int64_t)std::min(jj, (double)k);
+        a = std::fma((double)j, d, b);
+        k -= j;
+        continue;
+      }
+    }
+    a = b;
+  }
+  return a;
+}
+
+}  // namespace
+
 void get_bounds(int n, const double *x, double xmin[3], double xmax[3]) {
   if (n == 0) {
     for (int d = 0; d < 3; d++) xmin[d] = xmax[d] = 0;
     return;
   }
   for (int d = 0; d < 3; d++) xmin[d] = xmax[d] = x[d];
-  for (int i = 0; i < n; i++) {
+  const int T = n >= (1 << 16) ? host_threads() : 1;
+  if (T == 1) {
+    for (int i = 0; i < n; i++) {
+      for (int d = 0; d < 3; d++) {
+        double lo = xmin[d] - 1e-5, hi = xmax[d] + 1e-5, v = x[3 * (size_t)i + d];
+        xmin[d] = v < lo ? v : lo;
+        xmax[d] = v > hi ? v : hi;
+      }
+    }
+    return;
+  }
+  // runs of bodies scanned in parallel from +-inf, then chained in order
+  std::vector<std::array<double, 6>> run(T);
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; t++)
+    th.emplace_back([&, t] {
+      const int i0 = (int)((int64_t)n * t / T), i1 = (int)((int64_t)n * (t + 1) / T);
+      double lo3[3], hi3[3];
+      for (int d = 0; d < 3; d++) {
+        lo3[d] = HUGE_VAL;
+        hi3[d] = -HUGE_VAL;
+      }
+      for (int i = i0; i < i1; i++)
+        for (int d = 0; d < 3; d++) {
+          double lo = lo3[d] - 1e-5, hi = hi3[d] + 1e-5, v = x[3 * (size_t)i + d];
+          lo3[d] = v < lo ? v : lo;
+          hi3[d] = v > hi ? v : hi;
+        }
+      for (int d = 0; d < 3; d++) {
+        run[t][d] = lo3[d];
+        run[t][3 + d] = hi3[d];
+      }
+    });
+  for (auto &h : th) h.join();
+  for (int t = 0; t < T; t++) {
+    const int64_t k = (int64_t)n * (t + 1) / T - (int64_t)n * t / T;
     for (int d = 0; d < 3; d++) {
-      double lo = xmin[d] - 1e-5, hi = xmax[d] + 1e-5, v = x[3 * (size_t)i + d];
-      xmin[d] = v < lo ? v : lo;
-      xmax[d] = v > hi ? v : hi;
+      const double lo = drift(xmin[d], k, false), hi = drift(xmax[d], k, true);
+      xmin[d] = run[t][d] < lo ? run[t][d] : lo;
+      xmax[d] = run[t][3 + d] > hi ? run[t][3 + d] : hi;
     }
   }
 }

@@ -67,6 +67,34 @@ def test_plan_matches_oracle(dist, n, ncrit, theta, nspawn):
     assert s["m2l_pairs"] == int((ty == 0).sum()) and s["p2p_pairs"] == int((ty == 1).sum())
 
 
+@pytest.mark.parametrize("case", ["cube", "first_far", "tiny", "huge", "plummer"])
+def test_bounds_chained_scan_equals_sequential(case):
+    """BoundBox::getBounds (include/bound_box.h:16-23) widens the box by 1e-5
+    per body, rounding each step.  fmm_plan_build scans runs of bodies on
+    host threads (n >= 65536) and chains the runs with an exact k-fold shift;
+    the box must equal the oracle's one sequential scan bit for bit, also
+    where the drift of the first body decides it (crossing zero, small
+    binades, coordinates whose spacing exceeds the step)."""
+    F = _F()
+    rng = np.random.default_rng(11)
+    n = 70001 if case != "plummer" else 131072
+    if case == "plummer":
+        x, _ = F.make_bodies(n, "plummer1", 2)
+    else:
+        x = rng.uniform(-1, 1, (n, 3))
+    if case == "first_far":
+        x = 1.0 + np.abs(x)
+        x[0] = (-0.5, 0.7, 3.0517578125e-05)
+    elif case == "tiny":
+        x *= 1e-3
+    elif case == "huge":
+        x = x * 1e11 + 3e11
+    lo, hi = O.bounds(np.ascontiguousarray(x))
+    s = F.Plan(x, ncrit=64).stats()
+    assert np.array_equal(np.array(s["xmin"]), np.asarray(lo)), (s["xmin"], lo)
+    assert np.array_equal(np.array(s["xmax"]), np.asarray(hi)), (s["xmax"], hi)
+
+
 def test_plan_empty_and_duplicates():
     F = _F()
     pl = F.Plan(np.zeros((0, 3)))
