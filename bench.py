@@ -395,7 +395,7 @@ def graph_small(F, log, steps=50):
     return out
 
 
-def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0):
+def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, slots=5, host_threads=0):
     """Poisson arrivals into the request loop; latency = submit -> result on
     the server's clock; throughput over the whole stream; rel-L2 vs the FP64
     direct sum on 100 sampled targets of every `check_every`-th request.
@@ -413,8 +413,8 @@ def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0):
         % (name, nreq, int(sizes.sum()), sizes.mean(), w["rate"]))
     prec = F.FMM_MIXED if w["precision"] == "mixed" else F.FMM_FP64
     srv = F.Server(P=w["P"], theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], precision=prec,
-                   m2l_mode=m2l_mode, slots=3, batch_bodies=1 << 20, batch_requests=64,
-                   plan_on_device=plan_on_device)
+                   m2l_mode=m2l_mode, slots=slots, host_threads=host_threads, batch_bodies=1 << 20,
+                   batch_requests=64, plan_on_device=plan_on_device)
     for x, q in systems[:8]:   # warm-up: contexts allocate, kernels load
         srv.wait(srv.submit(x, q))
     base = srv.stats()

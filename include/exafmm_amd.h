@@ -345,7 +345,7 @@ typedef struct fmm_server fmm_server;
 
 typedef struct {
   int host_threads;       /* plan builders; 0: hardware threads - slots (max 16) */
-  int slots;              /* batches in flight (contexts/streams), 1..16 */
+  int slots;              /* batches in flight (contexts/streams), 1..16; default 5 */
   int64_t device_budget;  /* bytes the in-flight batches may occupy */
   int batch_bodies;       /* a batch closes at this many bodies ... */
   int batch_requests;     /* ... or this many requests */
