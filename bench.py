@@ -395,7 +395,8 @@ def graph_small(F, log, steps=50):
     return out
 
 
-def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, slots=5, host_threads=0):
+def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, slots=5, host_threads=0,
+                batch_requests=64):
     """Poisson arrivals into the request loop; latency = submit -> result on
     the server's clock; throughput over the whole stream; rel-L2 vs the FP64
     direct sum on 100 sampled targets of every `check_every`-th request.
@@ -414,7 +415,7 @@ def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, s
     prec = F.FMM_MIXED if w["precision"] == "mixed" else F.FMM_FP64
     srv = F.Server(P=w["P"], theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"], precision=prec,
                    m2l_mode=m2l_mode, slots=slots, host_threads=host_threads, batch_bodies=1 << 20,
-                   batch_requests=64, plan_on_device=plan_on_device)
+                   batch_requests=batch_requests, plan_on_device=plan_on_device)
     for x, q in systems[:8]:   # warm-up: contexts allocate, kernels load
         srv.wait(srv.submit(x, q))
     base = srv.stats()
@@ -670,6 +671,12 @@ def main():
         # the serving stream at its stated size: 2048 requests, N = 2^U[12,18]
         # cube or Plummer, P = 8, mixed, Poisson arrivals at 200 and 1000 req/s
         out["serving"] = {name: serve_bench(F, name, 0, 64, log) for name in ("serve-200", "serve-1000")}
+        # the same 200 req/s stream with every request planned on the device
+        # (tree, lists, M2L groups and tiles) and solved alone: the plan's
+        # arrays go device to device, no host merge or grouping
+        out["serving"]["serve-200-device-plan"] = dict(
+            serve_bench(F, "serve-200", 0, 64, log, plan_on_device=1, m2l_mode=F.M2L_TILES, batch_requests=1),
+            loop={"plan_on_device": 1, "m2l_mode": "tiles", "batch_requests": 1})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, q, w, plan, args.cpu_budget)
     print(json.dumps(out))

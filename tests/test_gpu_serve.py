@@ -77,6 +77,32 @@ def test_plans_built_on_the_device():
         assert _rel(td[:, 0], th[:, 0]) < MIX_TOL and _rel(td[:, 1:], th[:, 1:]) < MIX_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_planned_tiles_one_request_per_batch(precision):
+    """plan_on_device with M2L tiles and one request per batch: each request's
+    tree, lists, M2L groups and tiles are built on the device and uploaded
+    device to device.  Same results as the host-planned loop with tiles (FP64:
+    1e-12; mixed: the FP32 P2P sums each target's sources in list order,
+    which differs between device and host lists, so 1e-6 there) and equal
+    to a context solving the request alone from its own device plan (1e-12)."""
+    F = _F()
+    prec = F.FMM_MIXED if precision == "mixed" else F.FMM_FP64
+    systems = _systems(F, 10, 3)
+    host, _ = _serve(F, systems, prec, m2l_mode=F.M2L_TILES, slots=2, batch_requests=1)
+    dev, st = _serve(F, systems, prec, m2l_mode=F.M2L_TILES, slots=3, batch_requests=1, plan_on_device=1)
+    assert st["requests_done"] == 10 and st["max_batch_requests"] == 1
+    tol = MIX_TOL if precision == "fp64" else 1e-6
+    for (th, _), (td, _) in zip(host, dev):
+        assert _rel(td[:, 0], th[:, 0]) < tol and _rel(td[:, 1:], th[:, 1:]) < tol
+    for (td, _), (x, q) in zip(dev[:3], systems[:3]):
+        f = F.FMM(P=8, precision=prec, m2l_mode=F.M2L_TILES)
+        f.set_bodies_plan(F.Plan(x, lists_on=f), q)
+        f.evaluate()
+        t = f.targets(input_order=True)
+        f.close()
+        assert _rel(td[:, 0], t[:, 0]) < MIX_TOL and _rel(td[:, 1:], t[:, 1:]) < MIX_TOL
+
+
 def test_memory_admission_serialises_batches():
     """A budget below any footprint admits a batch only when none is in
     flight: batch intervals never overlap."""
