@@ -10,6 +10,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -105,6 +107,9 @@ struct fmm_ctx {
   DBuf dip;  // dipole partial sums
   DBuf ew_b, ew_t, ew_k, ew_s;  // fmm_ewald: bodies, targets, wave vectors, structure factors
   DBuf op_d, op_i;  // fmm_kernel_*: doubles (bodies, centres, expansions) and ints of one call
+  // pinned host staging of the upload's large copies (grow-only, hipHostMalloc)
+  void *pin = nullptr;
+  size_t pin_cap = 0;
 };
 
 struct fmm_plan {
@@ -169,6 +174,53 @@ int dcopy(fmm_ctx *c, DBuf &b, const fmmdev::DevArray &a) {
   int rc = ensure(c, b, a.bytes);
   if (rc) return rc;
   if (a.bytes) HIPCHK(hipMemcpyAsync(b.p, a.p, a.bytes, hipMemcpyDeviceToDevice, c->st));
+  return FMM_OK;
+}
+
+// host arrays (pointer, bytes) to device buffers through the context's pinned
+// staging: the arrays are packed into it by host threads (large sizes), then
+// copied by DMA from pinned memory, which runs several times the rate of a
+// pageable copy.  The staging is reused by the next call: the caller
+// synchronises the stream before that.
+int upload_staged(fmm_ctx *c, std::initializer_list<std::tuple<DBuf *, const void *, size_t>> parts) {
+  size_t total = 0;
+  for (const auto &pt : parts) total += (std::get<2>(pt) + 255) / 256 * 256;
+  if (total > c->pin_cap) {
+    if (c->pin) HIPCHK(hipHostFree(c->pin));
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    HIPCHK(hipHostMalloc(&c->pin, total + total / 4, hipHostMallocDefault));
+    c->pin_cap = total + total / 4;
+  }
+  // pack: runs of about 16 MB per host thread
+  std::vector<std::tuple<char *, const char *, size_t>> runs;
+  size_t off = 0;
+  for (const auto &pt : parts) {
+    const size_t b = std::get<2>(pt);
+    const char *src = static_cast<const char *>(std::get<1>(pt));
+    for (size_t o = 0; o < b; o += (size_t)16 << 20)
+      runs.emplace_back(static_cast<char *>(c->pin) + off + o, src + o, std::min(b - o, (size_t)16 << 20));
+    off += (b + 255) / 256 * 256;
+  }
+  const int nth = (int)std::min<size_t>(runs.size(), 8);
+  if (nth > 1) {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t] {
+        for (size_t r = t; r < runs.size(); r += nth) std::memcpy(std::get<0>(runs[r]), std::get<1>(runs[r]), std::get<2>(runs[r]));
+      });
+    for (auto &h : th) h.join();
+  } else {
+    for (auto &r : runs) std::memcpy(std::get<0>(r), std::get<1>(r), std::get<2>(r));
+  }
+  off = 0;
+  for (const auto &pt : parts) {
+    const size_t b = std::get<2>(pt);
+    int rc = ensure(c, *std::get<0>(pt), b);
+    if (rc) return rc;
+    if (b) HIPCHK(hipMemcpyAsync(std::get<0>(pt)->p, static_cast<char *>(c->pin) + off, b, hipMemcpyHostToDevice, c->st));
+    off += (b + 255) / 256 * 256;
+  }
   return FMM_OK;
 }
 
@@ -330,6 +382,7 @@ void fmm_destroy(fmm_ctx *c) {
     fmmdev::plan_scratch_release(c->st);  // plan builds on this stream
     (void)hipStreamDestroy(c->st);
   }
+  if (c->pin) (void)hipHostFree(c->pin);
   delete c;  // the device buffers are freed here, with the device current
 }
 
@@ -766,11 +819,15 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   }
   // bodies as given and the tree's permutation; gathered into tree order
   // (SoA) on the device
-  if ((rc = upload(c, c->xin, x, 3 * (size_t)n)) || (rc = upload(c, c->qin, q, (size_t)n)) ||
-      (rc = upload(c, c->perm, t.perm.data(), t.perm.size())) || (rc = ensure(c, c->x, sizeof(double) * n)) ||
+  if ((rc = upload_staged(c, {{&c->xin, x, sizeof(double) * 3 * (size_t)n},
+                               {&c->qin, q, sizeof(double) * (size_t)n},
+                               {&c->perm, t.perm.data(), sizeof(int) * t.perm.size()}})) ||
+      (rc = ensure(c, c->x, sizeof(double) * n)) ||
       (rc = ensure(c, c->y, sizeof(double) * n)) || (rc = ensure(c, c->z, sizeof(double) * n)) ||
-      (rc = ensure(c, c->q, sizeof(double) * n)))
+      (rc = ensure(c, c->q, sizeof(double) * n))) {
+    (void)hipStreamSynchronize(c->st);  // copies from the staging may be in flight
     return rc;
+  }
   HIPCHK(fmmdev::launch_gather_bodies(n, c->perm.as<int>(), c->xin.as<double>(), c->qin.as<double>(),
                                       c->x.as<double>(), c->y.as<double>(), c->z.as<double>(), c->q.as<double>(),
                                       c->st));

@@ -127,7 +127,11 @@ __global__ void pair_key_kernel(int64_t np, const int *__restrict__ ptgt, const 
   const unsigned long long mask = (1ull << kHashBits) - 1;
   unsigned long long h = mix64(key) & mask;
   for (int probe = 0; probe <= (int)mask; probe++) {
-    const unsigned long long prev = atomicCAS(&table[h], kEmptyKey, key);
+    // a slot changes once, empty -> key: a plain read that shows a key is
+    // final (most pairs find theirs so, without an atomic on a hot slot);
+    // one that shows empty may be stale, and the CAS decides
+    unsigned long long prev = table[h];
+    if (prev == kEmptyKey) prev = atomicCAS(&table[h], kEmptyKey, key);
     if (prev == kEmptyKey || prev == key) {
       slot[p] = (int)h;
       return;

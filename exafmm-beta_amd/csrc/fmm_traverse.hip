@@ -149,14 +149,21 @@ __global__ void key_src_kernel(const unsigned long long *__restrict__ key, size_
   if (i < n) src[i] = (int)(unsigned)(key[i] & 0xffffffffull);
 }
 
+// grid-stride over the pairs, one atomic per block (256 threads)
 __global__ void body_pairs_kernel(const int2 *__restrict__ pr, size_t n, const int *__restrict__ nbody,
                                   unsigned long long *__restrict__ out) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned long long part[4];
   unsigned long long v = 0;
-  if (i < n) v = (unsigned long long)nbody[pr[i].x] * (unsigned long long)nbody[pr[i].y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v += (unsigned long long)nbody[pr[i].x] * (unsigned long long)nbody[pr[i].y];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(out, t);
+  }
 }
 
 // device scratch from the plan builders' per-stream cache (plan_scratch_get)
@@ -519,8 +526,8 @@ hipError_t device_lists(int ncell, const double *X, const double *R, const int *
   if (np > 0) {
     TCHK(bp.need(sizeof(unsigned long long)));
     TCHK(hipMemsetAsync(bp.p, 0, sizeof(unsigned long long), s));
-    body_pairs_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(Pp.as<int2>(), np, dnb.as<int>(),
-                                                                    bp.as<unsigned long long>());
+    body_pairs_kernel<<<(unsigned)std::min<size_t>((np + 255) / 256, 2048), 256, 0, s>>>(
+        Pp.as<int2>(), np, dnb.as<int>(), bp.as<unsigned long long>());
     TCHK(hipGetLastError());
     unsigned long long v = 0;
     TCHK(hipMemcpyAsync(&v, bp.p, sizeof v, hipMemcpyDeviceToHost, s));
