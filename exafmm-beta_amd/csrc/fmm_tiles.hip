@@ -199,7 +199,13 @@ __global__ void seg_stats_kernel(int64_t np, const unsigned long long *__restric
   const int sg = segid[i] - 1;
   if (i == 0 || (key[i] >> 8) != (key[i - 1] >> 8)) sstart[sg] = (int)i;
   if (i + 1 == np || (key[i + 1] >> 8) != (key[i] >> 8)) slen[sg] = (int)(i + 1);  // end, made a length below
-  if (i + 1 == np || key[i + 1] != key[i]) atomicMax(&smost[sg], (int)(i - runstart[i] + 1));
+  // the longest run of one target; runs of one pair cannot raise the slice
+  // count max(ceil(m / SW), longest run) above ceil(m / SW) >= 1, so only
+  // longer runs (a target's several reflections of one class) take an atomic
+  if (i + 1 == np || key[i + 1] != key[i]) {
+    const int len = (int)(i - runstart[i] + 1);
+    if (len > 1) atomicMax(&smost[sg], len);
+  }
 }
 
 __global__ void seg_slices_kernel(int nseg, const int *__restrict__ sstart, int *__restrict__ slen,

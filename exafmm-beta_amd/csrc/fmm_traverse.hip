@@ -135,13 +135,27 @@ __global__ __launch_bounds__(256) void trav_round_kernel(const int2 *__restrict_
   }
 }
 
+// sort keys (target, source) and the per-target counts; a wavefront's lanes
+// holding one target in a row (the traversal emits a target's pairs
+// together) add their count with one atomic
 __global__ void pair_keys_kernel(const int2 *__restrict__ pr, size_t n, unsigned long long *__restrict__ key,
                                  int *__restrict__ count) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int2 p = pr[i];
-  key[i] = ((unsigned long long)(unsigned)p.x << 32) | (unsigned)p.y;
-  atomicAdd(&count[p.x], 1);
+  const int lane = threadIdx.x & 63;
+  int t = -1;
+  if (i < n) {
+    const int2 p = pr[i];
+    key[i] = ((unsigned long long)(unsigned)p.x << 32) | (unsigned)p.y;
+    t = p.x;
+  }
+  const int prev = __shfl_up(t, 1, 64);
+  const bool head = lane == 0 || prev != t;
+  const unsigned long long heads = __ballot(head);
+  if (head && t >= 0) {
+    const unsigned long long above = lane == 63 ? 0ull : heads >> (lane + 1);
+    const int len = above ? __ffsll((long long)above) : 64 - lane;
+    atomicAdd(&count[t], len);
+  }
 }
 
 __global__ void key_src_kernel(const unsigned long long *__restrict__ key, size_t n, int *__restrict__ src) {
