@@ -180,9 +180,10 @@ int dcopy(fmm_ctx *c, DBuf &b, const fmmdev::DevArray &a) {
 // host arrays (pointer, bytes) to device buffers through the context's pinned
 // staging: the arrays are packed into it by host threads (large sizes), then
 // copied by DMA from pinned memory, which runs several times the rate of a
-// pageable copy.  The staging is reused by the next call: the caller
-// synchronises the stream before that.
+// pageable copy.  The staging is reused by the next call, which first waits
+// for the stream (the copies of the previous call read it).
 int upload_staged(fmm_ctx *c, std::initializer_list<std::tuple<DBuf *, const void *, size_t>> parts) {
+  HIPCHK(hipStreamSynchronize(c->st));
   size_t total = 0;
   for (const auto &pt : parts) total += (std::get<2>(pt) + 255) / 256 * 256;
   if (total > c->pin_cap) {
@@ -475,8 +476,15 @@ int fmm_plan_build_device(fmm_ctx *c, fmm_plan **out, int n, const double *x, in
       if (n > 0) {
         double X0[3], R0;
         fmm::root_box(pl->xmin, pl->xmax, X0, R0);
+        // the bodies go up through the context's pinned staging into its
+        // upload buffer (scratch between uploads)
+        int rc = upload_staged(c, {{&c->xin, x, sizeof(double) * 3 * (size_t)n}});
+        if (rc) {
+          (void)hipStreamSynchronize(c->st);
+          return rc;
+        }
         HIPCHK(fmmdev::device_tree(n, x, ncrit, X0, R0, t.perm, t.iparent, t.ichild, t.nchild, t.ibody, t.nbody,
-                                   t.level, t.X, t.R, c->st));
+                                   t.level, t.X, t.R, c->st, c->xin.as<double>()));
         t.ncell = (int)t.nbody.size();
         fmm::finish_tree(t);
       } else {

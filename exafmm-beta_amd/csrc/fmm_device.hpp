@@ -211,11 +211,13 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
                                PlanArrays *keep = nullptr);
 // The octree of n bodies (host array x, AoS) built on the device: the same
 // cells, numbering and body order as fmm::build_tree for the root box (X0,
-// R0) (fmm_traverse.hip); outputs are host arrays.
+// R0) (fmm_traverse.hip); outputs are host arrays.  dx_in: the same bodies
+// already on the device (then x is not read), written on stream s.
 hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], double R0, std::vector<int> &perm,
                        std::vector<int> &iparent, std::vector<int> &ichild, std::vector<int> &nchild,
                        std::vector<int> &ibody, std::vector<int> &nbody, std::vector<int> &level,
-                       std::vector<double> &X, std::vector<double> &R, hipStream_t s);
+                       std::vector<double> &X, std::vector<double> &R, hipStream_t s,
+                       const double *dx_in = nullptr);
 // The interaction lists of a free-space tree built on the device (the dual
 // traversal breadth first, fmm_traverse.hip): CSR lists sorted by (target,
 // source), the P2P body-pair count; the cell arrays are host arrays.  With

@@ -677,7 +677,7 @@ hipError_t fetch_tiles(const PlanArrays &d, fmm::M2LTiles &T, hipStream_t s) {
 hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], double R0, std::vector<int> &perm,
                        std::vector<int> &iparent, std::vector<int> &ichild, std::vector<int> &nchild,
                        std::vector<int> &ibody, std::vector<int> &nbody, std::vector<int> &level,
-                       std::vector<double> &X, std::vector<double> &R, hipStream_t s) {
+                       std::vector<double> &X, std::vector<double> &R, hipStream_t s, const double *dx_in) {
   perm.resize(n);
   if (n == 0) {
     for (auto *v : {&iparent, &ichild, &nchild, &ibody, &nbody, &level}) v->clear();
@@ -688,12 +688,16 @@ hipError_t device_tree(int n, const double *x, int ncrit, const double X0[3], do
   const unsigned nb = (unsigned)((n + 255) / 256);
   using u64 = unsigned long long;
   Buf dx(s), kh(s), kl(s), ka(s), kb(s), idx(s), idx2(s), tmp(s);
-  TCHK(dx.need(sizeof(double) * 3 * (size_t)n));
-  TCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+  const double *xd = dx_in;
+  if (!xd) {
+    TCHK(dx.need(sizeof(double) * 3 * (size_t)n));
+    TCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+    xd = dx.as<double>();
+  }
   for (Buf *b : {&kh, &kl, &ka, &kb}) TCHK(b->need(8 * (size_t)n));
   TCHK(idx.need(4 * (size_t)n));
   TCHK(idx2.need(4 * (size_t)n));
-  tree_keys_kernel<<<nb, 256, 0, s>>>(n, dx.as<double>(), X0[0], X0[1], X0[2], R0, kh.as<u64>(), kl.as<u64>(),
+  tree_keys_kernel<<<nb, 256, 0, s>>>(n, xd, X0[0], X0[1], X0[2], R0, kh.as<u64>(), kl.as<u64>(),
                                       idx.as<int>());
   TCHK(hipGetLastError());
   // stable sort by the path, least significant part first: by levels 21-29
