@@ -225,6 +225,16 @@ int upload_staged(fmm_ctx *c, std::initializer_list<std::tuple<DBuf *, const voi
   return FMM_OK;
 }
 
+// a device array taken over by a context buffer (both hipMalloc'd)
+void adopt(DBuf &b, fmmdev::DevArray &a) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = a.p;
+  b.cap = a.bytes;
+  a.p = nullptr;
+  a.bytes = 0;
+  a.device = -1;
+}
+
 // the host copies of a device-built plan's lists and tiles, for host-side
 // uses (merging, host grouping, list export, another device); the plan's
 // contents do not change, only where they are held
@@ -932,28 +942,53 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   c->nchunk = 0;
   c->ntile = 0;
   fmm::M2LTiles tiles;
+  fmmdev::PlanArrays loc;  // groups and tiles built on the device by this upload
+  bool plan_tiles = false;  // tiles taken from the plan
   if (c->use_mfma) {
     // groups built with the plan on the device carry no chunks (grp_ptr is
     // {0}): the chunk path builds its own
     if (pl.groups_kind == (c->mixed ? FMM_MIXED : FMM_FP64) &&
         (c->use_tiles || pl.groups.grp_ptr.size() == (size_t)pl.groups.ngroup + 1)) {
       c->groups = std::move(pl.groups);
-      if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) tiles = std::move(pl.tiles);
+      if (pl.tiles.TT == fmmdev::m2l_tile_targets(c->p.P)) {
+        tiles = std::move(pl.tiles);
+        plan_tiles = true;
+      }
     } else {
-      if (dev_lists) {  // host grouping reads them
-        const hipError_t e = fmmdev::fetch_lists(*pdev, c->lists, c->st);
+      bool done = false;
+      if (c->use_tiles && c->p.images == 0 && L.nm2l > 0) {
+        // free space on tiles: the groups and tiles built on the device from
+        // the lists just uploaded (the host build's groups, classes, tiles
+        // and slices), their pair arrays left there; irregular trees and
+        // offsets out of the key's range fall back to the host build
+        const hipError_t e = fmmdev::device_groups_tiles(t, L, c->mixed, fmmdev::m2l_tile_targets(c->p.P), 16,
+                                                         fmmdev::kM2LItemSlices, c->groups, tiles, done, c->st,
+                                                         &loc, c->m2l_src.as<int>());
         if (e != hipSuccess) {
           (void)hipStreamSynchronize(c->st);
           HIPCHK(e);
         }
+        if (!done) {
+          c->groups = fmm::M2LGroups();
+          tiles = fmm::M2LTiles();
+        }
       }
-      c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed,
-                                        c->p.images > 0 ? c->p.cycle : nullptr);
+      if (!done) {
+        if (dev_lists) {  // host grouping reads them
+          const hipError_t e = fmmdev::fetch_lists(*pdev, c->lists, c->st);
+          if (e != hipSuccess) {
+            (void)hipStreamSynchronize(c->st);
+            HIPCHK(e);
+          }
+        }
+        c->groups = fmm::build_m2l_groups(t, L, fmmdev::kM2LChunkPairs, c->mixed,
+                                          c->p.images > 0 ? c->p.cycle : nullptr);
+      }
     }
     const fmm::M2LGroups &G = c->groups;
     UP(g_dx, G.dx) UP(r_ptr, G.rest_ptr) UP(r_src, G.rest_src) UP(r_tgt, G.rest_targets)
     if (c->use_tiles) {
-      const bool dt = tiles.TT > 0 && dev_tiles;  // the plan's tiles, their pair arrays on this device
+      const bool dt = plan_tiles && dev_tiles;  // the plan's tiles, their pair arrays on this device
       if (tiles.TT == 0) tiles = fmm::build_m2l_tiles(t, G, fmmdev::m2l_tile_targets(c->p.P), 16);
       c->ntile = (int)(tiles.item.size() / 4);  // work items
       // the tile kernel holds an item's slice descriptors one per lane
@@ -963,6 +998,10 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
       UP(t_tgt, tiles.tgt) UP(t_ptr, tiles.item)
       if (dt) {
         DC(t_slice, pdev->t_slice) DC(t_src, pdev->t_src) DC(t_lt, pdev->t_lt)
+      } else if (loc.tiles) {  // built here on the device
+        adopt(c->t_slice, loc.t_slice);
+        adopt(c->t_src, loc.t_src);
+        adopt(c->t_lt, loc.t_lt);
       } else {
         UP(t_slice, tiles.slice) UP(t_src, tiles.src) UP(t_lt, tiles.lt)
       }

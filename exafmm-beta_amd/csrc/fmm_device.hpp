@@ -205,10 +205,11 @@ hipError_t fetch_tiles(const PlanArrays &d, fmm::M2LTiles &T, hipStream_t s);
 // the tile path reads are filled (no chunks).  With keep, the pair sources
 // are read from keep->m2l_src when keep->lists (else from L.m2l_src), and
 // the tiles' slices, pair sources and pair targets are left in keep instead
-// of T (keep->tiles).
+// of T (keep->tiles).  dsrc_in: the lists' source column already on the
+// device (stream s), read instead of L.m2l_src.
 hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_level, int TT, int SW, int seg,
                                fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s,
-                               PlanArrays *keep = nullptr);
+                               PlanArrays *keep = nullptr, const int *dsrc_in = nullptr);
 // The octree of n bodies (host array x, AoS) built on the device: the same
 // cells, numbering and body order as fmm::build_tree for the root box (X0,
 // R0) (fmm_traverse.hip); outputs are host arrays.  dx_in: the same bodies

@@ -257,7 +257,7 @@ __global__ void slice_desc_kernel(int nseg, const unsigned long long *__restrict
 
 hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_level, int TT, int SW, int seg,
                                fmm::M2LGroups &G, fmm::M2LTiles &T, bool &done, hipStream_t s,
-                               PlanArrays *keep) {
+                               PlanArrays *keep, const int *dsrc_in) {
   done = false;
   if (keep) {
     keep->tiles = false;
@@ -277,10 +277,11 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   TCHK(dlev.need(4 * (size_t)nc));
   TCHK(dpar.need(4 * (size_t)nc));
   TCHK(dptr.need(4 * ((size_t)nc + 1)));
-  // the pair sources: the lists' device column when the plan keeps one
-  const bool dev_src = keep && keep->lists && keep->m2l_src.bytes == 4 * (size_t)np;
+  // the pair sources: given on the device, or the lists' device column when
+  // the plan keeps one, else uploaded from L
+  const bool dev_src = dsrc_in || (keep && keep->lists && keep->m2l_src.bytes == 4 * (size_t)np);
   if (!dev_src) TCHK(dsrc.need(4 * (size_t)np));
-  const int *psrc = dev_src ? keep->m2l_src.as<int>() : dsrc.as<int>();
+  const int *psrc = dsrc_in ? dsrc_in : dev_src ? keep->m2l_src.as<int>() : dsrc.as<int>();
   TCHK(dtgt.need(4 * (size_t)np));
   TCHK(hipMemcpyAsync(dX.p, t.X.data(), sizeof(double) * 3 * (size_t)nc, hipMemcpyHostToDevice, s));
   TCHK(hipMemcpyAsync(dlev.p, t.level.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, s));
