@@ -395,8 +395,8 @@ def graph_small(F, log, steps=50):
     return out
 
 
-def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, slots=5, host_threads=0,
-                batch_requests=64):
+def serve_bench(F, name, nreq, check_every, log, plan_on_device=0, m2l_mode=0, slots=8, host_threads=0,
+                batch_requests=1):
     """Poisson arrivals into the request loop; latency = submit -> result on
     the server's clock; throughput over the whole stream; rel-L2 vs the FP64
     direct sum on 100 sampled targets of every `check_every`-th request.

@@ -106,7 +106,10 @@ void builder_loop(fmm_server *s) {
     fmm_plan *pl = nullptr;
     int rc = bc ? fmm_plan_build_device(bc, &pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn)
                 : fmm_plan_build(&pl, r->n, r->x.data(), s->p.ncrit, s->p.theta, s->p.nspawn);
-    if (rc == FMM_OK) rc = fmm_plan_prepare(pl, &s->p);
+    // M2L groups for the slots' contexts; free-space tiles are left to the
+    // upload, which builds them on the device (for a batch of several
+    // requests the merged forest is grouped on the host there)
+    if (rc == FMM_OK && !(s->p.m2l_mode == FMM_M2L_TILES && s->p.images == 0)) rc = fmm_plan_prepare(pl, &s->p);
     fmm_stats st{};
     if (rc == FMM_OK) fmm_plan_stats(pl, &st);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -263,10 +266,11 @@ extern "C" {
 
 void fmm_server_default_config(fmm_server_config *cfg) {
   cfg->host_threads = 0;
-  cfg->slots = 5;  // measured best of 3-8 on 16 host cores (builders take the rest)
+  cfg->slots = 8;  // with one request per batch on tiles: 673 / 715 / 746 req/s at 6 / 8 / 10 slots on
+                   // 16 host cores (builders take the rest; at 10 they fall behind)
   cfg->device_budget = (int64_t)32 << 30;
   cfg->batch_bodies = 1 << 20;
-  cfg->batch_requests = 64;
+  cfg->batch_requests = 1;  // a batch of several is merged and grouped on the host: slower
   cfg->batch_delay_ms = 0.0;
   cfg->plan_on_device = 0;
 }
@@ -290,12 +294,18 @@ int fmm_server_create(fmm_server **out, const fmm_params *p, const fmm_server_co
   fmm_server *s = new (std::nothrow) fmm_server();
   if (!s) return FMM_ERR_MEMORY;
   s->p = *p;
-  // AUTO: grouped chunks, not tiles.  The loop is bound by host planning, and
-  // the tiles cost about a fifth of a request's planning (25 ms of one core
-  // at 64k bodies); at 1000 req/s offered, 2^12-2^18-body requests, chunks
-  // served 356 req/s and tiles 305 req/s (the GPU busy 22% and 9%)
-  // (split-FP16 runs on tiles only)
-  if (s->p.m2l_mode == FMM_M2L_AUTO && s->p.precision != FMM_MIXED16) s->p.m2l_mode = FMM_M2L_GROUPED;
+  // AUTO, one request per batch: target tiles.  The builders then plan only
+  // the tree and lists; a slot's upload builds the groups and tiles on the
+  // device (grouped chunks are built on the host by the builders, about half
+  // of a request's planning).  At 1000 req/s offered, 2^12-2^18-body
+  // requests: tiles 715 req/s, chunks 580 req/s (8 and 5 slots).  Batches of
+  // several requests: grouped chunks, as a merged forest's tiles are grouped
+  // on the host, where mixed precision groups by level across systems of
+  // different radii and runs the pairs left over on the per-target kernel
+  // (results then depend on the batch, about 1e-8).  Split-FP16 runs on
+  // tiles only.
+  if (s->p.m2l_mode == FMM_M2L_AUTO && s->p.precision != FMM_MIXED16)
+    s->p.m2l_mode = c.batch_requests == 1 ? FMM_M2L_TILES : FMM_M2L_GROUPED;
   s->cfg = c;
   s->t0 = clock_ms();
   for (int i = 0; i < c.host_threads; i++) s->builders.emplace_back(builder_loop, s);

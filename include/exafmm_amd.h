@@ -338,17 +338,23 @@ int fmm_get_lists(fmm_ctx *c, int *m2l_ptr, int *m2l_src, int *p2p_ptr, int *p2p
  * or batch_requests requests -- once their device footprint fits
  * device_budget beside the batches in flight, and evaluates them as one
  * forest (one launch per phase for the whole batch).  Batches of different
- * slots overlap on the GPU.  With m2l_mode AUTO the loop's M2L runs as grouped
- * chunks (FMM_M2L_GROUPED): no tiles to plan, since planning bounds it.  A system's result does not depend on its batch
- * beyond the order of FP64 atomic additions into its local expansions. */
+ * slots overlap on the GPU.  By default a batch holds one request: a batch of
+ * several is merged, then grouped on the host, which measured slower than
+ * more slots working at once.  With m2l_mode AUTO the loop's M2L runs on
+ * target tiles when batches hold one request (a slot's upload builds their
+ * groups on the device; the builders plan tree and lists only), and as
+ * grouped chunks when they may hold several.  A system's result does not
+ * depend on its batch beyond the order of FP64 atomic additions into its
+ * local expansions (for mixed precision on explicit FMM_M2L_TILES with
+ * batches of several requests it does, at about 1e-8). */
 typedef struct fmm_server fmm_server;
 
 typedef struct {
   int host_threads;       /* plan builders; 0: hardware threads - slots (max 16) */
-  int slots;              /* batches in flight (contexts/streams), 1..16; default 5 */
+  int slots;              /* batches in flight (contexts/streams), 1..16; default 8 */
   int64_t device_budget;  /* bytes the in-flight batches may occupy */
   int batch_bodies;       /* a batch closes at this many bodies ... */
-  int batch_requests;     /* ... or this many requests */
+  int batch_requests;     /* ... or this many requests (default 1) */
   double batch_delay_ms;  /* hold a batch open up to this long for more requests (0: never) */
   int plan_on_device;     /* 1: each builder builds the tree and lists on the device
                            * (fmm_plan_build_device, one context and stream per builder) */

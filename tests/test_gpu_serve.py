@@ -42,7 +42,12 @@ def _serve(F, systems, precision, m2l_mode=0, **cfg):
     return res, st
 
 
-@pytest.mark.parametrize("precision,mode", [("fp64", 0), ("mixed", 0), ("mixed", 2)])
+# mode 0 (AUTO) runs tiles alone (one request per batch) and grouped chunks
+# in the mix: FP64 only, where the two agree to the order of the sums.  Mixed
+# precision on explicit tiles is not batch-invariant (a merged forest's
+# by-level groups leave pairs of other radii to the per-target kernel), which
+# is why AUTO takes chunks for batches of several requests.
+@pytest.mark.parametrize("precision,mode", [("fp64", 0), ("fp64", 3), ("mixed", 2), ("fp64", 2)])
 def test_results_equal_alone_and_in_mix(precision, mode):
     F = _F()
     prec = F.FMM_MIXED if precision == "mixed" else F.FMM_FP64
