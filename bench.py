@@ -542,29 +542,25 @@ def main():
         # launches): it is timed apart, the second build is the figure
         fd = F.FMM(P=P, theta=w["theta"], ncrit=w["ncrit"], nspawn=w["nspawn"])
         t0 = time.perf_counter()
-        dplan = F.Plan(x, ncrit=w["ncrit"], theta=w["theta"], nspawn=w["nspawn"], lists_on=fd)
-        fd.set_bodies_plan(dplan, q)
+        fd.set_bodies_device(x, q)
         fd.synchronize()
         first_ms = 1e3 * (time.perf_counter() - t0)
-        del dplan
         t0 = time.perf_counter()
-        dplan = F.Plan(x, ncrit=w["ncrit"], theta=w["theta"], nspawn=w["nspawn"], lists_on=fd)
-        fd.set_bodies_plan(dplan, q)
+        fd.set_bodies_device(x, q)   # fmm_set_bodies_device: plan on the device, uploaded without a copy
         fd.synchronize()
         t1 = time.perf_counter()
-        sd, su = dplan.stats(), fd.stats()
-        del dplan
+        sd = fd.stats()
         fd.close()
         dplan_ms = {"bounds_ms": sd["host_bounds_ms"], "tree_ms": sd["host_tree_ms"],
                     "lists_ms": sd["host_lists_ms"], "groups_tiles_ms": sd["plan_groups_ms"],
-                    "upload_ms": su["host_upload_ms"], "total_ms": 1e3 * (t1 - t0), "first_build_total_ms": first_ms,
+                    "upload_ms": sd["host_upload_ms"], "total_ms": 1e3 * (t1 - t0), "first_build_total_ms": first_ms,
                     "host_tree_ms": st0["host_tree_ms"], "host_lists_ms": st0["host_lists_ms"],
                     "same_counts": all(sd[k] == st0[k] for k in ("ncell", "m2l_pairs", "p2p_pairs",
                                                                  "p2p_body_pairs"))}
         log("[bench] plan on the device: tree %.1f ms (host %.1f), lists %.1f ms (host %.1f), groups and tiles "
             "%.1f ms, upload %.1f ms; %.0f ms in all (first build %.0f ms); same counts %s"
             % (sd["host_tree_ms"], st0["host_tree_ms"], sd["host_lists_ms"], st0["host_lists_ms"],
-               sd["plan_groups_ms"], su["host_upload_ms"], dplan_ms["total_ms"], first_ms, dplan_ms["same_counts"]))
+               sd["plan_groups_ms"], sd["host_upload_ms"], dplan_ms["total_ms"], first_ms, dplan_ms["same_counts"]))
     r = solve_and_time(F, w, plan, x, q, w["precision"], args.steps, args.warmup, log, args.m2l_mode)
     st = r["st"]
     kernels = r["kernels"]

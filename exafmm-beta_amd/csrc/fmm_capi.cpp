@@ -775,6 +775,20 @@ int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q) {
   return upload_plan(c, std::move(pl), x, q);
 }
 
+int fmm_set_bodies_device(fmm_ctx *c, int n, const double *x, const double *q) {
+  if (!c) return FMM_ERR_ARG;
+  if (n < 0 || (n > 0 && (!x || !q))) FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies_device: bad n or NULL array");
+  if (!c->st) FMM_FAIL(FMM_ERR_STATE, "context was not created successfully");
+  if (c->p.images != 0) FMM_FAIL(FMM_ERR_UNSUPPORTED, "fmm_set_bodies_device: free space only (images = 0)");
+  fmm_plan *pl = nullptr;
+  int rc = fmm_plan_build_device(c, &pl, n, x, c->p.ncrit, c->p.theta, c->p.nspawn);
+  if (rc != FMM_OK) return rc;
+  // the plan's contents move into the context (no copy of the tree or lists)
+  rc = upload_plan(c, std::move(*pl), x, q);
+  fmm_plan_destroy(pl);
+  return rc;
+}
+
 int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const double *q) {
   if (!c || !pl) return FMM_ERR_ARG;
   if (pl->n > 0 && (!x || !q)) FMM_FAIL(FMM_ERR_ARG, "fmm_set_bodies_plan: NULL array");

@@ -23,7 +23,7 @@ SYMBOLS = [
     "fmm_default_params", "fmm_create", "fmm_destroy", "fmm_last_error", "fmm_set_bodies",
     "fmm_set_charges", "fmm_upward", "fmm_p2p", "fmm_m2l", "fmm_downward", "fmm_evaluate",
     "fmm_synchronize", "fmm_get_targets", "fmm_get_expansions", "fmm_get_stats", "fmm_get_tree",
-    "fmm_get_lists", "fmm_set_bodies_plan", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
+    "fmm_get_lists", "fmm_set_bodies_plan", "fmm_set_bodies_device", "fmm_plan_build", "fmm_plan_destroy", "fmm_plan_stats",
     "fmm_plan_tree", "fmm_plan_lists", "fmm_make_bodies",
     "fmm_direct", "fmm_set_tree", "fmm_plan_from_tree", "fmm_plan_build_groups", "fmm_plan_merge",
     "fmm_server_default_config", "fmm_server_create", "fmm_server_submit", "fmm_server_wait",
@@ -130,6 +130,7 @@ def load():
         L.fmm_get_tree.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
         L.fmm_get_lists.argtypes = [vp, _ip, _ip, _ip, _ip]
         L.fmm_set_bodies_plan.argtypes = [vp, vp, _dp, _dp]
+        L.fmm_set_bodies_device.argtypes = [vp, ctypes.c_int, _dp, _dp]
         L.fmm_plan_build.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_int]
         L.fmm_plan_build_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int, _dp, ctypes.c_int,
@@ -244,6 +245,16 @@ class FMM:
         self.n = len(plan.x)
         self._check(load().fmm_set_bodies_plan(self._h, plan._h, _ptr(plan.x), _ptr(q)),
                     "fmm_set_bodies_plan")
+
+    def set_bodies_device(self, x, q):
+        """fmm_set_bodies_device: tree, lists (and tile groups) built on the
+        device, uploaded without an intermediate host copy."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if len(q) != len(x):
+            raise ValueError("x and q lengths differ")
+        self.n = len(x)
+        self._check(load().fmm_set_bodies_device(self._h, self.n, _ptr(x), _ptr(q)), "fmm_set_bodies_device")
 
     def set_bodies(self, x, q):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

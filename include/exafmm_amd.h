@@ -179,6 +179,13 @@ int fmm_set_bodies(fmm_ctx *c, int n, const double *x, const double *q);
 /* Same, with a plan built by fmm_plan_build for these x (the context copies
  * what it needs; the plan stays owned by the caller). */
 int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const double *q);
+/* fmm_set_bodies with the tree, the lists and (tile path) the M2L groups and
+ * tiles built on c's device: fmm_plan_build_device with c's ncrit, theta and
+ * nspawn, then the upload, which takes the plan's contents without a copy.
+ * Free space only (FMM_ERR_UNSUPPORTED for images > 0).  Each cell's lists
+ * are ordered by source cell (fmm_set_bodies keeps the reference's visiting
+ * order); results agree with fmm_set_bodies' to the order of the sums. */
+int fmm_set_bodies_device(fmm_ctx *c, int n, const double *x, const double *q);
 
 /* Drop-in path for callers that build their own tree (exafmm-beta's
  * BuildTree::buildTree, include/build_tree.h:186-209): bodies already in tree

@@ -202,3 +202,30 @@ def test_device_plan_arrays_held_on_the_device():
     rel = np.linalg.norm(out[1] - out[0], axis=0) / np.linalg.norm(out[0], axis=0)
     assert np.all(rel < 1e-12), rel
     _same_pairs(mh.lists(), md.lists())
+
+
+def test_set_bodies_device():
+    """fmm_set_bodies_device: the device plan uploaded without a copy.  The
+    same solve as fmm_plan_build_device + fmm_set_bodies_plan (1e-14: split
+    tiles add their parts into L atomically) and as fmm_set_bodies (1e-12);
+    the timings of the plan are in the context's stats; periodic contexts
+    are refused."""
+    F = _F()
+    x, q = F.make_bodies(30000, "plummer1", 7)
+    f = F.FMM(P=10)
+    out = []
+    f.set_bodies_device(x, q)
+    s = f.stats()
+    for load in (lambda: None, lambda: f.set_bodies_plan(F.Plan(x, lists_on=f), q), lambda: f.set_bodies(x, q)):
+        load()
+        f.evaluate()
+        out.append(f.targets())
+    f.close()
+    assert s["m2l_groups"] > 0 and s["host_tree_ms"] > 0 and s["plan_groups_ms"] > 0
+    for b, tol in ((out[1], 1e-14), (out[2], 1e-12)):
+        rel = np.linalg.norm(out[0] - b, axis=0) / np.linalg.norm(b, axis=0)
+        assert np.all(rel < tol), rel
+    g = F.FMM(P=6, images=1, cycle=2 * np.pi)
+    with pytest.raises(F.FMMError):
+        g.set_bodies_device(x, q)
+    g.close()
