@@ -142,6 +142,25 @@ __global__ void pair_key_kernel(int64_t np, const int *__restrict__ ptgt, const 
   slot[p] = -1;
 }
 
+// the filled slots of the hash table (any order: the host sorts them)
+__global__ void table_compact_kernel(int tsize, const unsigned long long *__restrict__ table,
+                                     unsigned long long *__restrict__ okey, int *__restrict__ oslot,
+                                     int *__restrict__ count) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= tsize) return;
+  const unsigned long long k = table[h];
+  if (k == kEmptyKey) return;
+  const int at = atomicAdd(count, 1);
+  okey[at] = k;
+  oslot[at] = h;
+}
+
+// slot2grp[gslot[g]] = g: the group of every filled slot
+__global__ void slot_group_kernel(int ng, const int *__restrict__ gslot, int *__restrict__ slot2grp) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ng) slot2grp[gslot[g]] = g;
+}
+
 // the octant of every target cell in its parent; sort keys (level, octant, cell)
 __global__ void target_key_kernel(int ncell, const int *__restrict__ ptr, const int *__restrict__ level,
                                   const int *__restrict__ iparent, const double *__restrict__ X,
@@ -302,19 +321,34 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
                                        by_level ? 1 : 0, dtab.as<unsigned long long>(), dslot.as<int>(),
                                        dbad.as<int>());
   TCHK(hipGetLastError());
-  std::vector<unsigned long long> tab(tsize);
-  int bad = 0;
-  TCHK(hipMemcpyAsync(tab.data(), dtab.p, 8 * tsize, hipMemcpyDeviceToHost, s));
+  // the filled slots come back (key, slot), not the whole table
+  DBufT dck(s), dcs(s), dcn(s);
+  TCHK(dck.need(8 * tsize));
+  TCHK(dcs.need(4 * tsize));
+  TCHK(dcn.need(4));
+  TCHK(hipMemsetAsync(dcn.p, 0, 4, s));
+  table_compact_kernel<<<(unsigned)((tsize + 255) / 256), 256, 0, s>>>((int)tsize, dtab.as<unsigned long long>(),
+                                                                        dck.as<unsigned long long>(), dcs.as<int>(),
+                                                                        dcn.as<int>());
+  TCHK(hipGetLastError());
+  int bad = 0, nfill = 0;
   TCHK(hipMemcpyAsync(&bad, dbad.p, 4, hipMemcpyDeviceToHost, s));
+  TCHK(hipMemcpyAsync(&nfill, dcn.p, 4, hipMemcpyDeviceToHost, s));
   TCHK(hipStreamSynchronize(s));
   if (bad) return hipSuccess;  // the host groups this plan
+  std::vector<unsigned long long> fk(nfill);
+  std::vector<int> fs(nfill);
+  if (nfill) {
+    TCHK(hipMemcpyAsync(fk.data(), dck.p, 8 * (size_t)nfill, hipMemcpyDeviceToHost, s));
+    TCHK(hipMemcpyAsync(fs.data(), dcs.p, 4 * (size_t)nfill, hipMemcpyDeviceToHost, s));
+    TCHK(hipStreamSynchronize(s));
+  }
   // groups in key order, classes in representative-key order (host)
-  std::vector<std::pair<unsigned long long, int>> keys;
-  for (size_t h = 0; h < tsize; h++)
-    if (tab[h] != kEmptyKey) keys.push_back({tab[h], (int)h});
+  std::vector<std::pair<unsigned long long, int>> keys(nfill);
+  for (int f = 0; f < nfill; f++) keys[f] = {fk[f], fs[f]};
   std::sort(keys.begin(), keys.end());
   const int ng = (int)keys.size();
-  std::vector<int> slot2grp(tsize, 0), gcanon(ng);
+  std::vector<int> gslot(ng), gcanon(ng);
   std::vector<unsigned char> grefl(ng);
   G = fmm::M2LGroups();
   G.ngroup = ng;
@@ -326,7 +360,7 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   const unsigned long long kmask = (1ull << kKBits) - 1;
   for (int g = 0; g < ng; g++) {
     const unsigned long long key = keys[g].first;
-    slot2grp[keys[g].second] = g;
+    gslot[g] = keys[g].second;
     const int e = (int)(key >> (3 * kKBits + 10));
     const unsigned long long lv = (key >> (3 * kKBits)) & 1023;
     long long k[3];
@@ -404,7 +438,13 @@ hipError_t device_groups_tiles(const fmm::Tree &t, const fmm::Lists &L, bool by_
   TCHK(ds2g.need(4 * tsize));
   TCHK(dgc.need(4 * (size_t)ng));
   TCHK(dgr.need((size_t)ng));
-  TCHK(hipMemcpyAsync(ds2g.p, slot2grp.data(), 4 * tsize, hipMemcpyHostToDevice, s));
+  {  // every filled slot's group, scattered on the device
+    DBufT dgs(s);
+    TCHK(dgs.need(4 * (size_t)ng));
+    TCHK(hipMemcpyAsync(dgs.p, gslot.data(), 4 * (size_t)ng, hipMemcpyHostToDevice, s));
+    slot_group_kernel<<<(unsigned)((ng + 255) / 256), 256, 0, s>>>(ng, dgs.as<int>(), ds2g.as<int>());
+    TCHK(hipGetLastError());
+  }
   TCHK(hipMemcpyAsync(dgc.p, gcanon.data(), 4 * (size_t)ng, hipMemcpyHostToDevice, s));
   TCHK(hipMemcpyAsync(dgr.p, grefl.data(), (size_t)ng, hipMemcpyHostToDevice, s));
   TCHK(dpk.need(8 * (size_t)np));
