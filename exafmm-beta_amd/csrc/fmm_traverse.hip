@@ -505,7 +505,7 @@ hipError_t device_lists(int ncell, const double *X, const double *R, const int *
   unsigned long long h[3];
   while (nf > 0) {
     if (nf > 0x7fffffff) return hipErrorInvalidValue;  // frontier beyond one launch's index range
-    // count: this round's M2L, P2P and frontier pairs
+    // count: the M2L, P2P and next-frontier pairs of this traversal round
     TCHK(hipMemsetAsync(cnt.p, 0, sizeof(h), s));
     const unsigned nb = (unsigned)((nf + 255) / 256);
     trav_round_kernel<false><<<nb, 256, 0, s>>>(F0.as<int2>(), (int)nf, t, theta, xper[0], xper[1], xper[2], nspawn,
