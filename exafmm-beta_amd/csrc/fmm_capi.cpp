@@ -439,7 +439,9 @@ void plan_stats(const fmm_plan *pl, fmm_stats &s) {
   s.m2l_groups = pl->groups_kind >= 0 ? pl->groups.ngroup : 0;  // groups built with the plan
 }
 
-int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q);
+// x_on_device: x is already in c->xin (fmm_set_bodies_device: the plan was
+// just built from it there)
+int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q, bool x_on_device = false);
 }  // namespace
 
 int fmm_make_bodies(int n, const char *dist, int seed, double *x, double *q) {
@@ -783,8 +785,9 @@ int fmm_set_bodies_device(fmm_ctx *c, int n, const double *x, const double *q) {
   fmm_plan *pl = nullptr;
   int rc = fmm_plan_build_device(c, &pl, n, x, c->p.ncrit, c->p.theta, c->p.nspawn);
   if (rc != FMM_OK) return rc;
-  // the plan's contents move into the context (no copy of the tree or lists)
-  rc = upload_plan(c, std::move(*pl), x, q);
+  // the plan's contents move into the context (no copy of the tree or
+  // lists); the bodies are still in c->xin from the tree build
+  rc = upload_plan(c, std::move(*pl), x, q, n > 0);
   fmm_plan_destroy(pl);
   return rc;
 }
@@ -801,7 +804,7 @@ int fmm_set_bodies_plan(fmm_ctx *c, const fmm_plan *pl, const double *x, const d
 }
 
 namespace {
-int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
+int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q, bool x_on_device) {
   HIPCHK(hipSetDevice(c->p.device));
   graph_reset(c);
   c->have_bodies = false;
@@ -851,9 +854,11 @@ int upload_plan(fmm_ctx *c, fmm_plan &&pl, const double *x, const double *q) {
   }
   // bodies as given and the tree's permutation; gathered into tree order
   // (SoA) on the device
-  if ((rc = upload_staged(c, {{&c->xin, x, sizeof(double) * 3 * (size_t)n},
-                               {&c->qin, q, sizeof(double) * (size_t)n},
-                               {&c->perm, t.perm.data(), sizeof(int) * t.perm.size()}})) ||
+  if ((rc = x_on_device ? upload_staged(c, {{&c->qin, q, sizeof(double) * (size_t)n},
+                                             {&c->perm, t.perm.data(), sizeof(int) * t.perm.size()}})
+                        : upload_staged(c, {{&c->xin, x, sizeof(double) * 3 * (size_t)n},
+                                            {&c->qin, q, sizeof(double) * (size_t)n},
+                                            {&c->perm, t.perm.data(), sizeof(int) * t.perm.size()}})) ||
       (rc = ensure(c, c->x, sizeof(double) * n)) ||
       (rc = ensure(c, c->y, sizeof(double) * n)) || (rc = ensure(c, c->z, sizeof(double) * n)) ||
       (rc = ensure(c, c->q, sizeof(double) * n))) {
